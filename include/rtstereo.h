@@ -1,0 +1,286 @@
+/*
+ * rtstereo.h -- C ABI of librtstereo, the MI355X (gfx950) dense stereo library.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream
+ * (hipStream_t passed as void*, NULL = the null stream), launches work
+ * asynchronously on that stream and returns an rts_status.  Nothing here
+ * allocates device memory unless its name says so (rts_malloc, the
+ * rts_pipeline_* and rts_server_* objects).
+ *
+ * Device data layouts
+ *   gray image     uint8  [H][W]
+ *   colour image   uint8  [H][W][3]  interleaved, as the reference's OpenCV
+ *                                    imread buffers (seq/Main.cpp:87-90)
+ *   cost volume    float  [D][H][W]  disparity-major: plane d is an H x W
+ *                                    image, so a wavefront's 64 lanes touch
+ *                                    64 consecutive x of one (d, y) row.
+ *                                    The reference's host volumes are
+ *                                    pixel-major cost[d + D*(y*W+x)]
+ *                                    (seq/PixelBasedCost.cpp:21); rts_vol_to_dm
+ *                                    / rts_vol_to_pm convert.
+ *   census         uint64 [nw][H][W] bit k of word k/64 is entry k of the
+ *                                    reference's bool[H][W][N] descriptor
+ *                                    (seq/CensusTransform.cpp:24-28)
+ *   arms           uint32 [H][W]     byte 0 left, 1 right, 2 top, 3 bottom
+ *                                    (the reference's HorzOffset/VertOffset
+ *                                    int pairs, seq/CrossBasedAggregation.cpp:102-115)
+ *   disparity      uint8  [H][W]
+ *
+ * Limits: 1 <= W, H; 1 <= D <= 256 (disparities are uint8); D <= W; window
+ * radii smaller than the image side they wrap around.
+ *
+ * Reference interfaces: each launcher names the seq/ function whose
+ * arithmetic it reproduces.  The per-stage C++ drop-in with the reference's
+ * exact signatures is declared in rtstereo_seq.h and is built on this ABI.
+ */
+#ifndef RTSTEREO_H
+#define RTSTEREO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int rts_status;
+enum {
+    RTS_OK = 0,
+    RTS_ERR_INVALID = 1,     /* bad argument or unsupported size */
+    RTS_ERR_HIP = 2,         /* HIP runtime error */
+    RTS_ERR_NOMEM = 3,       /* device allocation failed */
+    RTS_ERR_BUDGET = 4,      /* request exceeds the admission budget */
+    RTS_ERR_CANCELLED = 5,   /* request cancelled before it ran */
+    RTS_ERR_NOTREADY = 6     /* result not ready yet (polling) */
+};
+
+/* ------------------------------------------------------------ runtime */
+
+const char *rts_last_error(void);                /* thread-local message of the last failure */
+const char *rts_version(void);
+rts_status rts_device_count(int *n);
+rts_status rts_set_device(int device);
+rts_status rts_malloc(void **ptr, size_t bytes);
+rts_status rts_free(void *ptr);
+rts_status rts_host_alloc(void **ptr, size_t bytes);      /* pinned host memory */
+rts_status rts_host_free(void *ptr);
+rts_status rts_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+rts_status rts_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+rts_status rts_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+rts_status rts_fill_u8(void *dst, uint8_t value, size_t bytes, void *stream);  /* fill kernel, graph-safe */
+rts_status rts_stream_create(void **stream);
+rts_status rts_stream_destroy(void *stream);
+rts_status rts_stream_sync(void *stream);
+rts_status rts_device_sync(void);
+rts_status rts_event_create(void **ev);
+rts_status rts_event_destroy(void *ev);
+rts_status rts_event_record(void *ev, void *stream);
+rts_status rts_event_sync(void *ev);
+rts_status rts_event_elapsed_ms(void *start, void *stop, float *ms);
+rts_status rts_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* Pixel-major float volume [H*W][D] <-> disparity-major [D][H][W]. */
+rts_status rts_vol_to_dm(const float *pm, float *dm, int W, int H, int D, void *stream);
+rts_status rts_vol_to_pm(const float *dm, float *pm, int W, int H, int D, void *stream);
+
+/* Plain copy kernel over `bytes` (multiple of 16), used to calibrate the HBM
+ * counters and the achievable copy bandwidth. */
+rts_status rts_copy_kernel(const void *src, void *dst, size_t bytes, void *stream);
+
+/* ------------------------------------------------------------ matching costs */
+
+/* Direction of the target pixel: L2R pairs ref x with tar x-d, R2L with x+d;
+ * both wrap around the row. */
+enum { RTS_L2R = 0, RTS_R2L = 1 };
+
+/* Gray AD |ref - tar|.  seq/PixelBasedCost.cpp:13-37 pixelBasedL2R / pixelBasedR2L. */
+rts_status rts_ad_gray(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int dir, void *stream);
+
+/* Float AD.  seq/PixelBasedCost.cpp:39-69 pixelBasedCostL2R_Float / R2L_Float. */
+rts_status rts_ad_float(const float *ref, const float *tar, float *vol, int W, int H, int D, int dir, void *stream);
+
+/* Colour AD: sum of the three channel |differences|; truncated to `trunc`
+ * when use_trunc; times float(1/3) when norm.  seq/PixelBasedCost.cpp:72-158
+ * pixelBasedCostL2R_Color (both overloads) / pixelBasedR2L_Color. */
+rts_status rts_ad_color(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int dir,
+                        int use_trunc, float trunc, int norm, void *stream);
+
+/* Census descriptors packed into uint64 words ([nw][H][W], nw = ceil(N/64),
+ * N = (2hr+1)(2wr+1)).  seq/CensusTransform.cpp:11-37 calcCensusTrans. */
+rts_status rts_census(const uint8_t *img, uint64_t *words, int W, int H, int hr, int wr, void *stream);
+/* bool[H][W][N] <-> packed words (the reference's census layout). */
+rts_status rts_census_unpack(const uint64_t *words, uint8_t *bools, int W, int H, int N, void *stream);
+rts_status rts_census_pack(const uint8_t *bools, uint64_t *words, int W, int H, int N, void *stream);
+
+/* Hamming cost popcount(ref(x) ^ tar(x-d)).  seq/CensusTransform.cpp:39-62 calcHammingDist. */
+rts_status rts_hamming(const uint64_t *cref, const uint64_t *ctar, float *vol, int W, int H, int D, int nwords,
+                       void *stream);
+
+/* Rank transform, radius m.  seq/RankFilter.cpp:11-34 calculateRankFilt. */
+rts_status rts_rank(const uint8_t *img, uint8_t *out, int W, int H, int m, void *stream);
+
+/* 2-D float correlation with a (2hr+1)x(2wr+1) host mask (copied to the
+ * kernel argument block, at most 15x15 taps), |.|, clamped to 255.
+ * seq/ConvolveImage.cpp:11-34 convolveImage. */
+rts_status rts_convolve(const uint8_t *img, float *out, const float *mask_host, int W, int H, int hr, int wr,
+                        void *stream);
+
+/* NCC cost.  seq/NCCDisparity.cpp:11-73 calcNCCDispL2R.
+ * RTS_NCC_SEQ: the reference's float window sums in its order (bit-exact).
+ * RTS_NCC_INTEGRAL: exact integer window sums (sliding sums, int64) combined
+ *   in FP64, written as float.
+ * RTS_NCC_KEEP_FLAT: where the denominator is 0 leave vol untouched (the
+ *   reference); otherwise write 0. */
+enum { RTS_NCC_SEQ = 0, RTS_NCC_INTEGRAL = 1, RTS_NCC_KEEP_FLAT = 0x100 };
+rts_status rts_ncc(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int hr, int wr,
+                   int flags, void *stream);
+
+/* In place c1 <- (1-exp(-c1/l1)) + (1-exp(-c2/l2)) over n elements.
+ * seq/CombineCost.cpp:11-25 combineCost. */
+rts_status rts_combine(float *c1, const float *c2, float l1, float l2, size_t n, void *stream);
+
+/* ------------------------------------------------------------ aggregation */
+
+/* Box mean over (2r+1)^2 with wrapped borders, out of place, float sum in
+ * the reference's window order.  seq/ConstBlockAggregation.cpp:11-34 boxAggregate
+ * (whose in-place update is replaced by a true box filter). */
+rts_status rts_box(const float *in, float *out, int W, int H, int D, int r, void *stream);
+
+/* Proximity weights exp(-sqrt(dy^2+dx^2)/gp) into a device float[(2r+1)^2].
+ * seq/AdaptiveWeightAggregation.cpp:11-18 calcProxWeight. */
+rts_status rts_prox_weight(float *prox, float gamma_p, int r, void *stream);
+
+/* Per-pixel support weights float[H][W][(2r+1)^2] (the reference's layout).
+ * seq/AdaptiveWeightAggregation.cpp:33-71 computeWeights. */
+rts_status rts_compute_weights(float *weights, const float *prox, const uint8_t *lab, float gamma_c, int W, int H,
+                               int r, void *stream);
+
+/* Adaptive-weight aggregation from precomputed weights (the reference's
+ * dataflow), out of place.  dir RTS_L2R: target centre x-d,
+ * seq/AdaptiveWeightAggregation.cpp:73-111 calcAWCostL2R; dir RTS_R2L: the same
+ * formula with target centre x+d (the intended form of calcAWCostR2L,
+ * seq/AdaptiveWeightAggregation.cpp:113-151, which indexes the weights at the
+ * neighbour and updates in place). */
+rts_status rts_aw_weights(const float *wref, const float *wtar, const float *vol, float *out, int W, int H, int D,
+                          int r, int dir, void *stream);
+
+/* Adaptive-weight aggregation with the weights built on the fly from the two
+ * Lab images (no weight tensors).  precision: RTS_AW_FP32 reproduces the
+ * reference's float arithmetic; RTS_AW_FP16 uses half weights and products
+ * with float accumulation. */
+enum { RTS_AW_FP32 = 0, RTS_AW_FP16 = 1 };
+rts_status rts_aw(const uint8_t *lab_ref, const uint8_t *lab_tar, const float *vol, float *out, int W, int H, int D,
+                  int r, float gamma_c, float gamma_p, int precision, void *stream);
+
+/* Cross arms (left, right, top, bottom), at most L, colour threshold tao.
+ * seq/CrossBasedAggregation.cpp:12-115 calc_x_range / calc_y_range / findCross. */
+rts_status rts_find_cross(const uint8_t *rgb, uint32_t *arms, int W, int H, int L, int tao, void *stream);
+/* Packed arms <-> the reference's int HorzOffset[H][W][2], VertOffset[H][W][2]. */
+rts_status rts_arms_unpack(const uint32_t *arms, int *horz, int *vert, int W, int H, void *stream);
+rts_status rts_arms_pack(const int *horz, const int *vert, uint32_t *arms, int W, int H, void *stream);
+
+/* Serial float prefix sum along x of every (d, y) row, in place.
+ * seq/CrossBasedAggregation.cpp:117-125 aggregate_cost_horizontal. */
+rts_status rts_hprefix(float *vol, int W, int H, int D, void *stream);
+
+/* Vertical sweep of horizontal segments over the prefix volume P.
+ * seq/CrossBasedAggregation.cpp:127-169 cross_stereo_aggregation.
+ * RTS_CROSS_EXACT: float sums in the reference's row order (bit-exact);
+ * RTS_CROSS_FAST: segment sums prefix-summed down each column in FP64. */
+enum { RTS_CROSS_EXACT = 0, RTS_CROSS_FAST = 1 };
+rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W, int H,
+                       int D, int mode, void *stream);
+
+/* One full cross-based aggregation from a raw cost volume (no prefix volume
+ * in HBM): local row prefix, segment sums and column sums in one kernel.
+ * Results equal rts_hprefix + rts_cross_v(FAST) up to float rounding. */
+rts_status rts_cross_fused(const float *cost, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W,
+                           int H, int D, void *stream);
+
+/* ------------------------------------------------------------ disparity */
+
+/* Winner-takes-all over d of a float volume, optionally with the right-view
+ * disparity read off the volume's diagonal, D_R(x) = argbest_d vol(x+d, d),
+ * and the left-right check |D_L(x) - D_R(x-D_L(x))| <= tol in the same pass.
+ * seq/WTA.cpp:10-39 WTA / WTA_NCC and seq/CrossCheck.cpp:10-23 cross_check.
+ * dr and mask may be NULL (mask requires dr).
+ *   RTS_WTA_MAX:     argmax (WTA_NCC) instead of argmin
+ *   RTS_WTA_LITERAL: a pixel where no d>0 beats d=0 keeps the previous
+ *                    pixel's (raster order) disparity, as seq/WTA.cpp:12-15
+ *                    does; otherwise it gets 0.  Applies to D_L only. */
+enum { RTS_WTA_MAX = 1, RTS_WTA_LITERAL = 2 };
+rts_status rts_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags,
+                   int tol, void *stream);
+
+/* Left-right check of two disparity maps (tol 0 = the reference's equality).
+ * seq/CrossCheck.cpp:10-23 cross_check. */
+rts_status rts_cross_check(const uint8_t *dl, const uint8_t *dr, uint8_t *mask, int W, int H, int tol, void *stream);
+
+/* Histogram vote over each pixel's cross support.  literal=1 updates in
+ * place in raster order (later pixels see refined neighbours), as the
+ * reference does; literal=0 reads `disp` and writes `out`.
+ * seq/CrossBasedAggregation.cpp:171-212 disp_refinement. */
+rts_status rts_disp_refine(uint8_t *disp, uint8_t *out, const uint32_t *arms, int W, int H, int D, int literal,
+                           void *stream);
+
+/* ------------------------------------------------------------ pipelines */
+
+/* One stereo pipeline in the order of the reference driver
+ * (seq/Main.cpp:180-257): first cost [+ second cost + combine] ->
+ * aggregation -> WTA [-> right view + left-right check]. */
+enum {
+    RTS_COST_AD = 1,      /* colour AD, optional truncation (Main.cpp:181-187) */
+    RTS_COST_RANK = 2,    /* rank transform then gray AD (Main.cpp:188-193) */
+    RTS_COST_CENSUS = 3,  /* census + Hamming (Main.cpp:194-199) */
+    RTS_COST_LOG = 4,     /* LoG then float AD (Main.cpp:200-205) */
+    RTS_COST_NCC = 5      /* NCC similarity (Main.cpp:287, commented out there) */
+};
+enum { RTS_AGGR_NONE = 0, RTS_AGGR_BOX = 1, RTS_AGGR_AW = 2, RTS_AGGR_CROSS = 3 };
+
+typedef struct rts_pipeline_desc {
+    int width, height, disp;        /* W, H, D */
+    int cost1, cost2;               /* RTS_COST_*; cost2 = 0 for a single cost */
+    float lambda1, lambda2;         /* combine weights (cost2 != 0) */
+    int ad_trunc_enable;            /* AD truncation on/off */
+    float ad_trunc;
+    int ad_norm;                    /* AD times 1/3 */
+    int census_hr, census_wr;       /* census radii (4, 3 in the reference) */
+    int rank_m;                     /* rank radius */
+    int ncc_hr, ncc_wr;             /* NCC radii */
+    int aggr;                       /* RTS_AGGR_* */
+    int box_r;
+    int cross_L, cross_tao, cross_iters, cross_exact;
+    int aw_r; float aw_gamma_c, aw_gamma_p; int aw_precision;
+    int wta_literal;                /* reproduce the reference's WTA carry */
+    int lr_check;                   /* compute D_R and the mask */
+    int lr_tol;
+} rts_pipeline_desc;
+
+typedef struct rts_pipeline rts_pipeline;
+
+/* Fills `desc` with the reference driver's constants (seq/Main.cpp:102-131)
+ * for the given size. */
+void rts_pipeline_defaults(rts_pipeline_desc *desc, int W, int H, int D);
+/* Device bytes the pipeline keeps resident (volumes and per-pixel buffers). */
+size_t rts_pipeline_bytes(const rts_pipeline_desc *desc);
+rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out);
+rts_status rts_pipeline_destroy(rts_pipeline *p);
+/* Runs on device images: colour [H][W][3] each; gray [H][W] is derived on
+ * the device when left_gray/right_gray are NULL; Lab [H][W][3] must be given
+ * for adaptive-weight aggregation.  dl is required, dr and mask when lr_check. */
+rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
+                            const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
+                            const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask, void *stream);
+/* Device pointer of the final aggregated float volume of the last run
+ * (disparity-major), for inspection. */
+const float *rts_pipeline_volume(rts_pipeline *p);
+
+/* Gray conversion used by the pipelines (the reference gets gray from
+ * OpenCV's imread, seq/Main.cpp:72-73): gray = (29B + 150G + 77R + 128) >> 8
+ * on interleaved B,G,R bytes. */
+rts_status rts_rgb_to_gray(const uint8_t *rgb, uint8_t *gray, int W, int H, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTSTEREO_H */

@@ -1,0 +1,389 @@
+// Matching-cost kernels: AD (gray, float, colour), census + Hamming, rank,
+// LoG convolution, NCC and the exponential cost combination.
+//
+// Every volume kernel writes the disparity-major layout [D][H][W]: a block
+// covers 256 consecutive x of one row and walks a chunk of disparities, so
+// each store instruction writes 256 contiguous floats.
+#include "rts_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace rts {
+
+constexpr int XB = 256;   // threads per block along x
+constexpr int DCH = 16;   // disparities per block
+
+enum { AD_GRAY = 0, AD_FLOAT = 1, AD_COLOR = 2 };
+
+// seq/PixelBasedCost.cpp:13-158.  dir 0: tar at x-d (L2R); 1: tar at x+d (R2L).
+template <int KIND>
+__global__ void __launch_bounds__(XB) k_ad(const void *__restrict__ ref_, const void *__restrict__ tar_,
+                                           float *__restrict__ vol, int W, int H, int D, int dir, int use_trunc,
+                                           float trunc, int norm) {
+    const int x = blockIdx.x * XB + threadIdx.x;
+    const int y = blockIdx.y;
+    const int d0 = blockIdx.z * DCH;
+    if (x >= W) return;
+    const size_t row = (size_t)y * W;
+    const size_t pl = plane(W, H);
+    const int dend = min(D, d0 + DCH);
+    if (KIND == AD_GRAY) {
+        const uint8_t *ref = (const uint8_t *)ref_, *tar = (const uint8_t *)tar_;
+        const int r = ref[row + x];
+        for (int d = d0; d < dend; d++) {
+            const int xt = wrap1(dir ? x + d : x - d, W);
+            const int t = r - (int)tar[row + xt];
+            vol[d * pl + row + x] = (float)(t < 0 ? -t : t);
+        }
+    } else if (KIND == AD_FLOAT) {
+        const float *ref = (const float *)ref_, *tar = (const float *)tar_;
+        const float r = ref[row + x];
+        for (int d = d0; d < dend; d++) {
+            const int xt = wrap1(dir ? x + d : x - d, W);
+            float v = r - tar[row + xt];
+            if (v < 0) v = -v;
+            vol[d * pl + row + x] = v;
+        }
+    } else {
+        const uint8_t *ref = (const uint8_t *)ref_, *tar = (const uint8_t *)tar_;
+        const uint32_t r = pack_rgb(ref + 3 * (row + x));
+        const float third = (float)1 / (float)3;
+        for (int d = d0; d < dend; d++) {
+            const int xt = wrap1(dir ? x + d : x - d, W);
+            const int s = (int)sad3(r, pack_rgb(tar + 3 * (row + xt)));
+            float v = use_trunc ? (((float)s > trunc) ? trunc : (float)s) : (float)s;
+            if (norm) v = third * v;
+            vol[d * pl + row + x] = v;
+        }
+    }
+}
+
+// seq/CensusTransform.cpp:11-37.  Entry k = (dy+hr)*(2wr+1) + (dx+wr) is set
+// when the centre is strictly brighter than the wrapped neighbour; entries
+// are packed little-endian into 64-bit words, word-major [nw][H][W].
+__global__ void __launch_bounds__(256) k_census(const uint8_t *__restrict__ img, uint64_t *__restrict__ words, int W,
+                                                int H, int hr, int wr) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const uint8_t c = img[(size_t)y * W + x];
+    const size_t pl = plane(W, H);
+    uint64_t acc = 0;
+    int k = 0, w = 0;
+    for (int dy = -hr; dy <= hr; dy++) {
+        const size_t rr = (size_t)wrap1(y + dy, H) * W;
+        for (int dx = -wr; dx <= wr; dx++) {
+            acc |= (uint64_t)(c > img[rr + wrap1(x + dx, W)]) << k;
+            if (++k == 64) {
+                words[w * pl + (size_t)y * W + x] = acc;
+                acc = 0;
+                k = 0;
+                w++;
+            }
+        }
+    }
+    if (k) words[w * pl + (size_t)y * W + x] = acc;
+}
+
+__global__ void k_census_unpack(const uint64_t *__restrict__ words, uint8_t *__restrict__ bools, int P, int N) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    for (int k = 0; k < N; k++) bools[(size_t)N * p + k] = (uint8_t)((words[(size_t)(k >> 6) * P + p] >> (k & 63)) & 1u);
+}
+
+__global__ void k_census_pack(const uint8_t *__restrict__ bools, uint64_t *__restrict__ words, int P, int N) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int nw = (N + 63) / 64;
+    for (int w = 0; w < nw; w++) {
+        uint64_t acc = 0;
+        for (int k = 64 * w; k < min(N, 64 * w + 64); k++)
+            acc |= (uint64_t)(bools[(size_t)N * p + k] != 0) << (k & 63);
+        words[(size_t)w * P + p] = acc;
+    }
+}
+
+// seq/CensusTransform.cpp:39-62: popcount of the XOR of the descriptors.
+__global__ void __launch_bounds__(XB) k_hamming(const uint64_t *__restrict__ cref, const uint64_t *__restrict__ ctar,
+                                                float *__restrict__ vol, int W, int H, int D, int nw) {
+    const int x = blockIdx.x * XB + threadIdx.x;
+    const int y = blockIdx.y;
+    const int d0 = blockIdx.z * DCH;
+    if (x >= W) return;
+    const size_t row = (size_t)y * W, pl = plane(W, H);
+    const int dend = min(D, d0 + DCH);
+    if (nw == 1) {
+        const uint64_t r = cref[row + x];
+        for (int d = d0; d < dend; d++)
+            vol[d * pl + row + x] = (float)__popcll(r ^ ctar[row + wrap1(x - d, W)]);
+    } else {
+        for (int d = d0; d < dend; d++) {
+            const int xt = wrap1(x - d, W);
+            int n = 0;
+            for (int w = 0; w < nw; w++) n += __popcll(cref[w * pl + row + x] ^ ctar[w * pl + row + xt]);
+            vol[d * pl + row + x] = (float)n;
+        }
+    }
+}
+
+// seq/RankFilter.cpp:11-34: 1 + number of window pixels darker than the centre.
+__global__ void __launch_bounds__(256) k_rank(const uint8_t *__restrict__ img, uint8_t *__restrict__ out, int W,
+                                              int H, int m) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const uint8_t c = img[(size_t)y * W + x];
+    int n = 1;
+    for (int dy = -m; dy <= m; dy++) {
+        const size_t rr = (size_t)wrap1(y + dy, H) * W;
+        for (int dx = -m; dx <= m; dx++) n += img[rr + wrap1(x + dx, W)] < c;
+    }
+    out[(size_t)y * W + x] = (uint8_t)n;
+}
+
+struct ConvMask {
+    float m[225];
+};
+
+// seq/ConvolveImage.cpp:11-34: float products added in row-major window
+// order (contraction off keeps each product rounded), |.|, clamp to 255.
+__global__ void __launch_bounds__(256) k_convolve(const uint8_t *__restrict__ img, float *__restrict__ out, int W,
+                                                  int H, int hr, int wr, ConvMask mask) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int mw = 2 * wr + 1;
+    float acc = 0.0f;
+    for (int dy = -hr; dy <= hr; dy++) {
+        const size_t rr = (size_t)wrap1(y + dy, H) * W;
+        for (int dx = -wr; dx <= wr; dx++) {
+            const float prod = (float)img[rr + wrap1(x + dx, W)] * mask.m[(dy + hr) * mw + (dx + wr)];
+            acc += prod;
+        }
+    }
+    acc = acc > 0 ? acc : -acc;
+    out[(size_t)y * W + x] = acc > 255 ? 255.0f : acc;
+}
+
+// seq/NCCDisparity.cpp:11-73 in the reference's own float order: window
+// means as float sums / area, centred values, nom = sum a'b',
+// den = sqrt(sum a'a'b'b') accumulated as ((a'a')b')b'.
+__global__ void __launch_bounds__(256) k_ncc_seq(const uint8_t *__restrict__ ref, const uint8_t *__restrict__ tar,
+                                                 float *__restrict__ vol, int W, int H, int D, int hr, int wr,
+                                                 int keep_flat) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int d0 = blockIdx.z * DCH;
+    if (x >= W || y >= H) return;
+    const size_t pl = plane(W, H);
+    const float area = (float)((2 * hr + 1) * (2 * wr + 1));
+    float mr = 0.0f;
+    for (int dy = -hr; dy <= hr; dy++) {
+        const size_t rr = (size_t)wrap1(y + dy, H) * W;
+        for (int dx = -wr; dx <= wr; dx++) mr += (float)ref[rr + wrap1(x + dx, W)];
+    }
+    mr = __fdiv_rn(mr, area);
+    const int dend = min(D, d0 + DCH);
+    for (int d = d0; d < dend; d++) {
+        float mt = 0.0f;
+        for (int dy = -hr; dy <= hr; dy++) {
+            const size_t rr = (size_t)wrap1(y + dy, H) * W;
+            for (int dx = -wr; dx <= wr; dx++) mt += (float)tar[rr + wrap1(x + dx - d, W)];
+        }
+        mt = __fdiv_rn(mt, area);
+        float nom = 0.0f, den = 0.0f;
+        for (int dy = -hr; dy <= hr; dy++) {
+            const size_t rr = (size_t)wrap1(y + dy, H) * W;
+            for (int dx = -wr; dx <= wr; dx++) {
+                const float a = (float)ref[rr + wrap1(x + dx, W)] - mr;
+                const float b = (float)tar[rr + wrap1(x + dx - d, W)] - mt;
+                nom += a * b;
+                den += ((a * a) * b) * b;
+            }
+        }
+        den = __fsqrt_rn(den);
+        const size_t o = d * pl + (size_t)y * W + x;
+        if (den != 0.0f)
+            vol[o] = __fdiv_rn(nom, den);
+        else if (!keep_flat)
+            vol[o] = 0.0f;
+    }
+}
+
+// NCC from exact integer window sums.  With N the window area and a, b the
+// ref and (shifted) tar values:
+//   N*nom           = N*Sab - Sa*Sb
+//   N^4*den^2       = N^4*Saabb - 2N^3(Sb*Saab + Sa*Sabb)
+//                     + N^2(Sb^2*Saa + Sa^2*Sbb + 4*Sa*Sb*Sab) - 3N*Sa^2*Sb^2
+// both exact (the second in 128-bit integers), so den == 0 is decided
+// exactly; the quotient N*(N*nom)/sqrt(N^4 den^2) is formed in FP64.
+__global__ void __launch_bounds__(256) k_ncc_int(const uint8_t *__restrict__ ref, const uint8_t *__restrict__ tar,
+                                                 float *__restrict__ vol, int W, int H, int D, int hr, int wr,
+                                                 int keep_flat) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    const int d0 = blockIdx.z * DCH;
+    if (x >= W || y >= H) return;
+    const size_t pl = plane(W, H);
+    const long N = (long)(2 * hr + 1) * (2 * wr + 1);
+    long Sa = 0, Saa = 0;
+    for (int dy = -hr; dy <= hr; dy++) {
+        const size_t rr = (size_t)wrap1(y + dy, H) * W;
+        for (int dx = -wr; dx <= wr; dx++) {
+            const long a = ref[rr + wrap1(x + dx, W)];
+            Sa += a;
+            Saa += a * a;
+        }
+    }
+    const int dend = min(D, d0 + DCH);
+    for (int d = d0; d < dend; d++) {
+        long Sb = 0, Sbb = 0, Sab = 0, Saab = 0, Sabb = 0;
+        unsigned long Saabb = 0;
+        for (int dy = -hr; dy <= hr; dy++) {
+            const size_t rr = (size_t)wrap1(y + dy, H) * W;
+            for (int dx = -wr; dx <= wr; dx++) {
+                const long a = ref[rr + wrap1(x + dx, W)];
+                const long b = tar[rr + wrap1(x + dx - d, W)];
+                const long ab = a * b;
+                Sb += b;
+                Sbb += b * b;
+                Sab += ab;
+                Saab += ab * a;
+                Sabb += ab * b;
+                Saabb += (unsigned long)(ab * ab);
+            }
+        }
+        const __int128 n = N;
+        const __int128 X = n * n * n * n * (__int128)Saabb - 2 * n * n * n * ((__int128)Sb * Saab + (__int128)Sa * Sabb) +
+                           n * n * ((__int128)Sb * Sb * Saa + (__int128)Sa * Sa * Sbb + 4 * (__int128)Sa * Sb * Sab) -
+                           3 * n * (__int128)Sa * Sa * Sb * Sb;
+        const long nomN = N * Sab - Sa * Sb;
+        const size_t o = d * pl + (size_t)y * W + x;
+        if (X > 0)
+            vol[o] = (float)((double)N * (double)nomN / sqrt((double)X));
+        else if (!keep_flat)
+            vol[o] = 0.0f;
+    }
+}
+
+// seq/CombineCost.cpp:11-25.
+__global__ void k_combine(float *__restrict__ c1, const float *__restrict__ c2, float l1, float l2, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        const float a = 1.0f - expf(__fdiv_rn(-c1[i], l1));
+        const float b = 1.0f - expf(__fdiv_rn(-c2[i], l2));
+        c1[i] = a + b;
+    }
+}
+
+}  // namespace rts
+
+using namespace rts;
+
+extern "C" {
+
+static rts_status launch_ad(int kind, const void *ref, const void *tar, float *vol, int W, int H, int D, int dir,
+                            int use_trunc, float trunc, int norm, void *stream) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(dir == RTS_L2R || dir == RTS_R2L, "bad direction %d", dir);
+    RTS_CHECK(H <= 65535, "height %d too large", H);
+    dim3 grid(div_up(W, XB), H, div_up(D, DCH));
+    if (kind == AD_GRAY)
+        k_ad<AD_GRAY><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, 0, 0.f, 0);
+    else if (kind == AD_FLOAT)
+        k_ad<AD_FLOAT><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, 0, 0.f, 0);
+    else
+        k_ad<AD_COLOR><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, use_trunc, trunc, norm);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+rts_status rts_ad_gray(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int dir, void *stream) {
+    return launch_ad(AD_GRAY, ref, tar, vol, W, H, D, dir, 0, 0.f, 0, stream);
+}
+rts_status rts_ad_float(const float *ref, const float *tar, float *vol, int W, int H, int D, int dir, void *stream) {
+    return launch_ad(AD_FLOAT, ref, tar, vol, W, H, D, dir, 0, 0.f, 0, stream);
+}
+rts_status rts_ad_color(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int dir,
+                        int use_trunc, float trunc, int norm, void *stream) {
+    return launch_ad(AD_COLOR, ref, tar, vol, W, H, D, dir, use_trunc, trunc, norm, stream);
+}
+
+rts_status rts_census(const uint8_t *img, uint64_t *words, int W, int H, int hr, int wr, void *stream) {
+    RTS_CHECK(W >= 1 && H >= 1, "bad image size %dx%d", W, H);
+    RTS_CHECK(hr >= 0 && wr >= 0 && hr < H && wr < W, "census radii %d,%d do not fit the image", hr, wr);
+    dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
+    k_census<<<grid, block, 0, S(stream)>>>(img, words, W, H, hr, wr);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+rts_status rts_census_unpack(const uint64_t *words, uint8_t *bools, int W, int H, int N, void *stream) {
+    RTS_CHECK(W >= 1 && H >= 1 && N >= 1, "bad census size");
+    const int P = W * H;
+    k_census_unpack<<<div_up(P, 256), 256, 0, S(stream)>>>(words, bools, P, N);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+rts_status rts_census_pack(const uint8_t *bools, uint64_t *words, int W, int H, int N, void *stream) {
+    RTS_CHECK(W >= 1 && H >= 1 && N >= 1, "bad census size");
+    const int P = W * H;
+    k_census_pack<<<div_up(P, 256), 256, 0, S(stream)>>>(bools, words, P, N);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+rts_status rts_hamming(const uint64_t *cref, const uint64_t *ctar, float *vol, int W, int H, int D, int nwords,
+                       void *stream) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(nwords >= 1, "nwords %d", nwords);
+    RTS_CHECK(H <= 65535, "height %d too large", H);
+    dim3 grid(div_up(W, XB), H, div_up(D, DCH));
+    k_hamming<<<grid, XB, 0, S(stream)>>>(cref, ctar, vol, W, H, D, nwords);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+rts_status rts_rank(const uint8_t *img, uint8_t *out, int W, int H, int m, void *stream) {
+    RTS_CHECK(W >= 1 && H >= 1, "bad image size %dx%d", W, H);
+    RTS_CHECK(m >= 0 && m < W && m < H, "rank radius %d does not fit the image", m);
+    dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
+    k_rank<<<grid, block, 0, S(stream)>>>(img, out, W, H, m);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+rts_status rts_convolve(const uint8_t *img, float *out, const float *mask_host, int W, int H, int hr, int wr,
+                        void *stream) {
+    RTS_CHECK(W >= 1 && H >= 1, "bad image size %dx%d", W, H);
+    RTS_CHECK(hr >= 0 && wr >= 0 && hr < H && wr < W, "mask radii %d,%d do not fit the image", hr, wr);
+    RTS_CHECK((2 * hr + 1) * (2 * wr + 1) <= 225, "mask of %dx%d taps exceeds 15x15", 2 * hr + 1, 2 * wr + 1);
+    ConvMask m{};
+    for (int i = 0; i < (2 * hr + 1) * (2 * wr + 1); i++) m.m[i] = mask_host[i];
+    dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
+    k_convolve<<<grid, block, 0, S(stream)>>>(img, out, W, H, hr, wr, m);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+rts_status rts_ncc(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int hr, int wr,
+                   int flags, void *stream) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(hr >= 0 && wr >= 0 && hr < H && wr + D <= W, "NCC radii %d,%d do not fit the image", hr, wr);
+    const int mode = flags & 0xff;
+    const int keep = (flags & RTS_NCC_KEEP_FLAT) != 0;
+    RTS_CHECK(mode == RTS_NCC_SEQ || mode == RTS_NCC_INTEGRAL, "bad NCC mode %d", mode);
+    RTS_CHECK(div_up(D, DCH) <= 65535, "too many disparities");
+    dim3 grid(div_up(W, 64), div_up(H, 4), div_up(D, DCH)), block(64, 4);
+    if (mode == RTS_NCC_SEQ)
+        k_ncc_seq<<<grid, block, 0, S(stream)>>>(ref, tar, vol, W, H, D, hr, wr, keep);
+    else
+        k_ncc_int<<<grid, block, 0, S(stream)>>>(ref, tar, vol, W, H, D, hr, wr, keep);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+rts_status rts_combine(float *c1, const float *c2, float l1, float l2, size_t n, void *stream) {
+    if (!n) return RTS_OK;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 256 * 32);
+    k_combine<<<blocks, 256, 0, S(stream)>>>(c1, c2, l1, l2, n);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+}  // extern "C"

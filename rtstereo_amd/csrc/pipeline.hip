@@ -1,0 +1,409 @@
+// Stereo pipelines: the reference driver's stage order (seq/Main.cpp:180-257)
+// on resident device buffers, with the census/AD/combine/row-prefix front end
+// of cross-based aggregation fused into one kernel.
+#include "rts_internal.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+
+#pragma clang fp contract(off)
+
+namespace rts {
+
+// ---------------------------------------------------------------- fused front end
+
+// Cost of (x, y, d) = TC[popcount(census_L(x) ^ census_R(x-d))] + TA[sad(rgb_L(x), rgb_R(x-d))]
+// followed by the serial row prefix P[x] = P[x-1] + cost(x) along x
+// (seq/CrossBasedAggregation.cpp:117-125).  TC and TA hold the combined
+// cost terms 1 - exp(-c/lambda) computed on the host with the C library's
+// expf, so TC[h] + TA[s] is bit-identical to seq/CombineCost.cpp:11-25 applied to
+// the integer census and AD costs.  A single cost uses a table of its own
+// values and a zero table (adding +0.0f is exact).
+//
+// One block per image row, one lane per disparity: lane d walks x = 0..W-1
+// serially, so its additions happen in the reference's order.  The right
+// view's census and colour rows sit in LDS (lanes read consecutive x-d);
+// the left view's values at x are uniform across the block.  Each lane
+// stages FH_CH prefix values in LDS, then the wave stores its 64 rows of
+// FH_CH floats contiguously.
+constexpr int FH_CH = 32;
+constexpr int FH_MAXW = 4096;
+
+template <bool CENSUS, bool AD>
+__global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict__ cL, const uint64_t *__restrict__ cR,
+                                                     const uint8_t *__restrict__ rgbL, const uint8_t *__restrict__ rgbR,
+                                                     const float *__restrict__ tabC, const float *__restrict__ tabA,
+                                                     int nA, float *__restrict__ P, int W, int H, int D, int vec) {
+    extern __shared__ uint8_t smem[];
+    uint64_t *rowC = reinterpret_cast<uint64_t *>(smem);                        // W (census)
+    uint32_t *rowA = reinterpret_cast<uint32_t *>(rowC + (CENSUS ? W : 0));   // W (packed colour)
+    float *tC = reinterpret_cast<float *>(rowA + (AD ? W : 0));               // 65
+    float *tA = tC + 65;                                                       // nA
+    float *stage = tA + nA;                                                    // waves * 64 * (FH_CH+1)
+    const int y = blockIdx.x;
+    const size_t roff = (size_t)y * W;
+    for (int x = threadIdx.x; x < W; x += blockDim.x) {
+        if (CENSUS) rowC[x] = cR[roff + x];
+        if (AD) rowA[x] = pack_rgb(rgbR + 3 * (roff + x));
+    }
+    for (int i = threadIdx.x; i < 65; i += blockDim.x) tC[i] = CENSUS ? tabC[i] : 0.0f;
+    for (int i = threadIdx.x; i < nA; i += blockDim.x) tA[i] = tabA[i];
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int d = threadIdx.x;
+    float *st = stage + wave * 64 * (FH_CH + 1);
+    const size_t pl = plane(W, H);
+    float acc = 0.0f;
+    int xt = d == 0 ? 0 : W - d;   // (x - d) mod W at x = 0
+    for (int x0 = 0; x0 < W; x0 += FH_CH) {
+        const int n = min(FH_CH, W - x0);
+        if (d < D) {
+            for (int j = 0; j < n; j++) {
+                const int x = x0 + j;
+                float c;
+                if (CENSUS && AD)
+                    c = tC[__popcll(cL[roff + x] ^ rowC[xt])] + tA[sad3(pack_rgb(rgbL + 3 * (roff + x)), rowA[xt])];
+                else if (CENSUS)
+                    c = tC[__popcll(cL[roff + x] ^ rowC[xt])];
+                else
+                    c = tA[sad3(pack_rgb(rgbL + 3 * (roff + x)), rowA[xt])];
+                acc = (x == 0) ? c : acc + c;
+                st[lane * (FH_CH + 1) + j] = acc;
+                xt = (xt + 1 == W) ? 0 : xt + 1;
+            }
+        }
+        __syncthreads();
+        // Store the wave's 64 rows (its lanes' disparities) of n floats.
+        const int dbase = wave * 64;
+        if (n == FH_CH && vec) {
+            // 8 lanes x float4 per row, 8 rows per instruction.
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int rr = it * 8 + (lane >> 3), j = (lane & 7) * 4;
+                const int dd = dbase + rr;
+                if (dd < D) {
+                    const float *s4 = st + rr * (FH_CH + 1) + j;
+                    *reinterpret_cast<float4 *>(P + (size_t)dd * pl + roff + x0 + j) =
+                        make_float4(s4[0], s4[1], s4[2], s4[3]);
+                }
+            }
+        } else {
+            for (int i = lane; i < 64 * n; i += 64) {
+                const int rr = i / n, j = i - rr * n;
+                const int dd = dbase + rr;
+                if (dd < D) P[(size_t)dd * pl + roff + x0 + j] = st[rr * (FH_CH + 1) + j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The AD value the colour AD kernel produces for integer channel sum s.
+static float ad_value(int s, int use_trunc, float trunc, int norm) {
+    const float third = (float)1 / (float)3;
+    float v = use_trunc ? (((float)s > trunc) ? trunc : (float)s) : (float)s;
+    if (norm) v = third * v;
+    return v;
+}
+
+// 1 - exp(-c/lambda) as seq/CombineCost.cpp:17-18 computes it in float.
+static float combine_term(float c, float lambda) { return 1 - expf(-c / lambda); }
+
+}  // namespace rts
+
+using namespace rts;
+
+struct rts_pipeline {
+    rts_pipeline_desc d;
+    // per-pixel buffers
+    uint8_t *gray[2] = {nullptr, nullptr};
+    uint64_t *census[2] = {nullptr, nullptr};
+    uint32_t *arms[2] = {nullptr, nullptr};
+    uint8_t *rank[2] = {nullptr, nullptr};
+    float *logimg[2] = {nullptr, nullptr};
+    float *mask_dev = nullptr;   // LoG mask (unused: passed by value)
+    // volumes
+    float *vol[2] = {nullptr, nullptr};
+    float *final_vol = nullptr;
+    // tables for the fused front end
+    float *tabC = nullptr, *tabA = nullptr;
+    int nA = 0;
+    int census_words = 1;
+    bool fused_front = false;
+};
+
+namespace {
+
+const float kLoG5[25] = {0.0239f, 0.046f,   0.0499f, 0.046f,   0.0239f,  0.046f,  0.0061f, -0.0923f, 0.0061f,
+                         0.046f,  0.0499f,  -0.0923f, -0.3182f, -0.0923f, 0.0499f, 0.046f,  0.0061f, -0.0923f,
+                         0.0061f, 0.046f,   0.0239f, 0.046f,   0.0499f,  0.046f,  0.0239f};
+
+bool uses(const rts_pipeline_desc &d, int cost) { return d.cost1 == cost || d.cost2 == cost; }
+
+int dev_alloc(void **p, size_t n) {
+    if (!n) {
+        *p = nullptr;
+        return RTS_OK;
+    }
+    hipError_t e = hipMalloc(p, n);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return fail(RTS_ERR_NOMEM, "pipeline allocation of %zu bytes failed: %s", n, hipGetErrorString(e));
+    }
+    return RTS_OK;
+}
+
+int validate(const rts_pipeline_desc &d) {
+    if (int e = check_dims(d.width, d.height, d.disp)) return e;
+    RTS_CHECK(d.cost1 >= RTS_COST_AD && d.cost1 <= RTS_COST_NCC, "bad first cost %d", d.cost1);
+    RTS_CHECK(d.cost2 >= 0 && d.cost2 <= RTS_COST_LOG, "bad second cost %d", d.cost2);
+    RTS_CHECK(!(d.cost1 == RTS_COST_NCC && d.cost2 != 0), "NCC is a similarity and is not combined");
+    RTS_CHECK(d.aggr >= RTS_AGGR_NONE && d.aggr <= RTS_AGGR_CROSS, "bad aggregation %d", d.aggr);
+    RTS_CHECK(!(d.cost1 == RTS_COST_NCC && (d.aggr == RTS_AGGR_CROSS || d.aggr == RTS_AGGR_AW)),
+              "NCC pipelines use box or no aggregation");
+    RTS_CHECK(d.census_hr >= 0 && d.census_wr >= 0 && (2 * d.census_hr + 1) * (2 * d.census_wr + 1) <= 64,
+              "census window must have at most 64 entries");
+    RTS_CHECK(d.cross_iters >= 1 && d.cross_iters <= 16, "cross iterations %d out of range", d.cross_iters);
+    RTS_CHECK(d.cross_L >= 0 && d.cross_L <= 255, "cross arm limit %d out of range", d.cross_L);
+    RTS_CHECK(d.width <= FH_MAXW, "width %d exceeds %d", d.width, FH_MAXW);
+    return RTS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rts_pipeline_defaults(rts_pipeline_desc *d, int W, int H, int D) {
+    memset(d, 0, sizeof(*d));
+    d->width = W;
+    d->height = H;
+    d->disp = D;
+    d->cost1 = RTS_COST_AD;
+    d->cost2 = 0;
+    d->lambda1 = 10.0f;
+    d->lambda2 = 10.0f;
+    d->ad_trunc_enable = 0;
+    d->ad_trunc = 24.0f;      // seq/Main.cpp:128
+    d->ad_norm = 0;
+    d->census_hr = 4;         // seq/Main.cpp:107-108
+    d->census_wr = 3;
+    d->rank_m = 7;            // seq/Main.cpp:111
+    d->ncc_hr = 1;            // seq/Main.cpp:119-120
+    d->ncc_wr = 1;
+    d->aggr = RTS_AGGR_BOX;
+    d->box_r = 1;             // seq/Main.cpp:102
+    d->cross_L = 17;          // seq/CrossBasedAggregation.h:8-9
+    d->cross_tao = 20;
+    d->cross_iters = 1;
+    d->cross_exact = 1;
+    d->aw_r = 16;             // seq/Main.cpp:122-124
+    d->aw_gamma_c = 7.0f;
+    d->aw_gamma_p = 36.0f;
+    d->aw_precision = RTS_AW_FP32;
+    d->wta_literal = 0;
+    d->lr_check = 1;
+    d->lr_tol = 0;
+}
+
+size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
+    const size_t P = (size_t)d->width * d->height;
+    const size_t vol = P * d->disp * sizeof(float);
+    size_t b = 2 * vol + 2 * P;   // two volumes, gray
+    if (uses(*d, RTS_COST_CENSUS)) b += 2 * P * 8;
+    if (d->aggr == RTS_AGGR_CROSS) b += 2 * P * 4;
+    if (uses(*d, RTS_COST_RANK)) b += 2 * P;
+    if (uses(*d, RTS_COST_LOG)) b += 2 * P * 4;
+    return b + 8192;
+}
+
+rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out) {
+    RTS_CHECK(desc && out, "null argument");
+    *out = nullptr;
+    if (int e = validate(*desc)) return e;
+    rts_pipeline *p = new (std::nothrow) rts_pipeline();
+    RTS_CHECK(p != nullptr, "out of host memory");
+    p->d = *desc;
+    const rts_pipeline_desc &d = p->d;
+    const size_t P = (size_t)d.width * d.height;
+    const size_t V = P * d.disp * sizeof(float);
+    int st = RTS_OK;
+    auto A = [&](void **ptr, size_t n) {
+        if (st == RTS_OK) st = dev_alloc(ptr, n);
+    };
+    for (int v = 0; v < 2; v++) {
+        A((void **)&p->gray[v], P);
+        if (uses(d, RTS_COST_CENSUS)) A((void **)&p->census[v], P * 8);
+        if (d.aggr == RTS_AGGR_CROSS) A((void **)&p->arms[v], P * 4);
+        if (uses(d, RTS_COST_RANK)) A((void **)&p->rank[v], P);
+        if (uses(d, RTS_COST_LOG)) A((void **)&p->logimg[v], P * 4);
+        A((void **)&p->vol[v], V);
+    }
+    // Fused front end: census and/or colour AD (combined) feeding cross-based aggregation.
+    const bool front_ok = d.aggr == RTS_AGGR_CROSS &&
+                          ((d.cost1 == RTS_COST_CENSUS && (d.cost2 == 0 || d.cost2 == RTS_COST_AD)) ||
+                           (d.cost1 == RTS_COST_AD && (d.cost2 == 0 || d.cost2 == RTS_COST_CENSUS)));
+    p->fused_front = front_ok;
+    if (front_ok && st == RTS_OK) {
+        float hC[65], hA[766];
+        const bool comb = d.cost2 != 0;
+        const float lc = d.cost1 == RTS_COST_CENSUS ? d.lambda1 : d.lambda2;
+        const float la = d.cost1 == RTS_COST_AD ? d.lambda1 : d.lambda2;
+        for (int h = 0; h < 65; h++) hC[h] = comb ? combine_term((float)h, lc) : (float)h;
+        for (int s = 0; s < 766; s++) {
+            const float v = ad_value(s, d.ad_trunc_enable, d.ad_trunc, d.ad_norm);
+            hA[s] = comb ? combine_term(v, la) : v;
+        }
+        p->nA = 766;
+        A((void **)&p->tabC, sizeof(hC));
+        A((void **)&p->tabA, sizeof(hA));
+        if (st == RTS_OK) {
+            hipError_t e1 = hipMemcpy(p->tabC, hC, sizeof(hC), hipMemcpyHostToDevice);
+            hipError_t e2 = hipMemcpy(p->tabA, hA, sizeof(hA), hipMemcpyHostToDevice);
+            if (e1 != hipSuccess || e2 != hipSuccess) st = fail(RTS_ERR_HIP, "table upload failed");
+        }
+    }
+    if (st != RTS_OK) {
+        rts_pipeline_destroy(p);
+        return st;
+    }
+    *out = p;
+    return RTS_OK;
+}
+
+rts_status rts_pipeline_destroy(rts_pipeline *p) {
+    if (!p) return RTS_OK;
+    for (int v = 0; v < 2; v++) {
+        (void)hipFree(p->gray[v]);
+        (void)hipFree(p->census[v]);
+        (void)hipFree(p->arms[v]);
+        (void)hipFree(p->rank[v]);
+        (void)hipFree(p->logimg[v]);
+        (void)hipFree(p->vol[v]);
+    }
+    (void)hipFree(p->tabC);
+    (void)hipFree(p->tabA);
+    delete p;
+    return RTS_OK;
+}
+
+const float *rts_pipeline_volume(rts_pipeline *p) { return p ? p->final_vol : nullptr; }
+
+}  // extern "C"
+
+namespace {
+
+// Builds cost `which` into `vol` (the non-fused path).
+int build_cost(rts_pipeline *p, int which, const uint8_t *const rgb[2], const uint8_t *const gray[2], float *vol,
+               hipStream_t s) {
+    const rts_pipeline_desc &d = p->d;
+    const int W = d.width, H = d.height, D = d.disp;
+    switch (which) {
+        case RTS_COST_AD:
+            return rts_ad_color(rgb[0], rgb[1], vol, W, H, D, RTS_L2R, d.ad_trunc_enable, d.ad_trunc, d.ad_norm, s);
+        case RTS_COST_RANK:
+            for (int v = 0; v < 2; v++)
+                if (int e = rts_rank(gray[v], p->rank[v], W, H, d.rank_m, s)) return e;
+            return rts_ad_gray(p->rank[0], p->rank[1], vol, W, H, D, RTS_L2R, s);
+        case RTS_COST_CENSUS:
+            for (int v = 0; v < 2; v++)
+                if (int e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, s)) return e;
+            return rts_hamming(p->census[0], p->census[1], vol, W, H, D, 1, s);
+        case RTS_COST_LOG:
+            for (int v = 0; v < 2; v++)
+                if (int e = rts_convolve(gray[v], p->logimg[v], kLoG5, W, H, 2, 2, s)) return e;
+            return rts_ad_float(p->logimg[0], p->logimg[1], vol, W, H, D, RTS_L2R, s);
+        case RTS_COST_NCC:
+            return rts_ncc(gray[0], gray[1], vol, W, H, D, d.ncc_hr, d.ncc_wr, RTS_NCC_INTEGRAL, s);
+    }
+    return fail(RTS_ERR_INVALID, "bad cost %d", which);
+}
+
+}  // namespace
+
+extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
+                                       const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
+                                       const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask,
+                                       void *stream) {
+    RTS_CHECK(p != nullptr, "null pipeline");
+    const rts_pipeline_desc &d = p->d;
+    const int W = d.width, H = d.height, D = d.disp;
+    hipStream_t s = S(stream);
+    RTS_CHECK(left_rgb && right_rgb, "colour images are required");
+    RTS_CHECK(dl != nullptr, "dl is required");
+    RTS_CHECK(!d.lr_check || (dr && mask), "lr_check needs dr and mask");
+    RTS_CHECK(d.aggr != RTS_AGGR_AW || (left_lab && right_lab), "adaptive weights need Lab images");
+    const uint8_t *rgb[2] = {left_rgb, right_rgb};
+    const uint8_t *gray[2] = {left_gray, right_gray};
+    const bool need_gray = uses(d, RTS_COST_CENSUS) || uses(d, RTS_COST_RANK) || uses(d, RTS_COST_LOG) ||
+                           uses(d, RTS_COST_NCC);
+    for (int v = 0; v < 2; v++)
+        if (need_gray && !gray[v]) {
+            if (int e = rts_rgb_to_gray(rgb[v], p->gray[v], W, H, s)) return e;
+            gray[v] = p->gray[v];
+        }
+    float *cur = p->vol[0], *other = p->vol[1];
+    if (d.aggr == RTS_AGGR_CROSS)
+        for (int v = 0; v < 2; v++)
+            if (int e = rts_find_cross(rgb[v], p->arms[v], W, H, d.cross_L, d.cross_tao, s)) return e;
+    bool prefixed = false;
+    if (p->fused_front) {
+        const bool census = uses(d, RTS_COST_CENSUS), ad = uses(d, RTS_COST_AD);
+        if (census)
+            for (int v = 0; v < 2; v++)
+                if (int e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, s)) return e;
+        const int waves = div_up(D, 64);
+        const size_t shm = (size_t)W * ((census ? 8 : 0) + (ad ? 4 : 0)) + sizeof(float) * (65 + p->nA) +
+                           sizeof(float) * waves * 64 * (FH_CH + 1);
+        RTS_CHECK(shm <= 160 * 1024, "front-end LDS %zu too large", shm);
+        const int vec = (W % 4) == 0 && ((uintptr_t)cur % 16) == 0;
+        if (census && ad)
+            k_front_hscan<true, true><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
+                                                                 p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
+        else if (census)
+            k_front_hscan<true, false><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
+                                                                  p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
+        else
+            k_front_hscan<false, true><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
+                                                                  p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
+        RTS_LAUNCHED();
+        prefixed = true;
+    } else {
+        if (int e = build_cost(p, d.cost1, rgb, gray, cur, s)) return e;
+        if (d.cost2) {
+            if (int e = build_cost(p, d.cost2, rgb, gray, other, s)) return e;
+            if (int e = rts_combine(cur, other, d.lambda1, d.lambda2, (size_t)W * H * D, s)) return e;
+        }
+    }
+    switch (d.aggr) {
+        case RTS_AGGR_NONE:
+            break;
+        case RTS_AGGR_BOX:
+            if (int e = rts_box(cur, other, W, H, D, d.box_r, s)) return e;
+            std::swap(cur, other);
+            break;
+        case RTS_AGGR_AW:
+            if (int e = rts_aw(left_lab, right_lab, cur, other, W, H, D, d.aw_r, d.aw_gamma_c, d.aw_gamma_p,
+                               d.aw_precision, s))
+                return e;
+            std::swap(cur, other);
+            break;
+        case RTS_AGGR_CROSS:
+            for (int it = 0; it < d.cross_iters; it++) {
+                if (!prefixed)
+                    if (int e = rts_hprefix(cur, W, H, D, s)) return e;
+                prefixed = false;
+                if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D,
+                                        d.cross_exact ? RTS_CROSS_EXACT : RTS_CROSS_FAST, s))
+                    return e;
+                std::swap(cur, other);
+            }
+            break;
+    }
+    p->final_vol = cur;
+    int flags = (d.cost1 == RTS_COST_NCC ? RTS_WTA_MAX : 0) | (d.wta_literal ? RTS_WTA_LITERAL : 0);
+    return rts_wta(cur, dl, d.lr_check ? dr : nullptr, d.lr_check ? mask : nullptr, W, H, D, flags, d.lr_tol, s);
+}

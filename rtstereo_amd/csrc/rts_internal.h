@@ -1,0 +1,84 @@
+// Internal helpers shared by the HIP translation units of librtstereo.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "rtstereo.h"
+
+namespace rts {
+
+// Records a message for rts_last_error() (thread-local) and returns `code`.
+int fail(int code, const char *fmt, ...);
+
+// Maps a HIP error to RTS_ERR_HIP with the call site recorded.
+#define RTS_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ::rts::fail(RTS_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                               __FILE__, __LINE__);                                     \
+    } while (0)
+
+#define RTS_CHECK(cond, ...)                                                            \
+    do {                                                                                \
+        if (!(cond)) return ::rts::fail(RTS_ERR_INVALID, __VA_ARGS__);                  \
+    } while (0)
+
+// After a kernel launch: report a launch failure as RTS_ERR_HIP.
+#define RTS_LAUNCHED()                                                                  \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess)                                                           \
+            return ::rts::fail(RTS_ERR_HIP, "launch: %s (%s:%d)", hipGetErrorString(e_),  \
+                               __FILE__, __LINE__);                                     \
+    } while (0)
+
+inline hipStream_t S(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
+
+inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+// One-step toroidal wrap of an index that is at most one period out of range
+// (the reference's border rule for every window and for x-d).
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+// Full modulo for tile-staging indices that may lie several periods away on
+// narrow images (only in-period entries are ever consumed).
+__device__ __forceinline__ int wrapn(int v, int n) {
+    v %= n;
+    return v < 0 ? v + n : v;
+}
+
+// Plane stride of a disparity-major volume [D][H][W].
+__host__ __device__ __forceinline__ size_t plane(int W, int H) { return (size_t)W * (size_t)H; }
+
+// Validates W, H, D against the limits every entry point shares.
+int check_dims(int W, int H, int D);
+
+// Colour AD helper: |a-b| summed over the three bytes of packed 0x00BBGGRR
+// pixels (v_sad_u8 does four byte |differences| plus an accumulator).
+__device__ __forceinline__ uint32_t sad3(uint32_t a, uint32_t b) { return __builtin_amdgcn_sad_u8(a, b, 0u); }
+
+__device__ __forceinline__ uint32_t pack_rgb(const uint8_t *p) {
+    return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16);
+}
+
+// Maximum per-channel |difference| of two packed pixels.
+__device__ __forceinline__ int max_chan_delta(uint32_t a, uint32_t b) {
+    int m = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int t = (int)((a >> (8 * c)) & 255u) - (int)((b >> (8 * c)) & 255u);
+        t = t < 0 ? -t : t;
+        m = t > m ? t : m;
+    }
+    return m;
+}
+
+// Packed arm layout: byte 0 left, 1 right, 2 top, 3 bottom.
+__device__ __forceinline__ int arm_l(uint32_t a) { return (int)(a & 255u); }
+__device__ __forceinline__ int arm_r(uint32_t a) { return (int)((a >> 8) & 255u); }
+__device__ __forceinline__ int arm_t(uint32_t a) { return (int)((a >> 16) & 255u); }
+__device__ __forceinline__ int arm_b(uint32_t a) { return (int)(a >> 24); }
+
+}  // namespace rts

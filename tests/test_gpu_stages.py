@@ -1,0 +1,220 @@
+"""Per-stage parity of the HIP kernels against the CPU oracle.
+
+Integer stages and the float stages that keep the reference's operation
+order are checked bit-exactly; stages that reorder float arithmetic are
+checked against the FP64 oracle with the tolerance written in the test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from rtstereo_amd import gen
+from conftest import dm, pm
+
+pytestmark = pytest.mark.gpu
+
+from rtstereo_amd import stages as S  # noqa: E402  (loads the HIP library; fails loudly if missing)
+from rtstereo_amd import _lib as L  # noqa: E402
+
+SIZES = [(40, 24, 8, 7), (29, 21, 5, 8), (67, 41, 13, 9), (130, 70, 64, 10)]
+
+
+def _pair(W, H, D, seed):
+    print(f"[case] {W}x{H} D={D} seed={seed}", flush=True)
+    return gen.make_pair(W, H, D, 4, 2.5, seed)
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES)
+def test_ad(W, H, D, seed):
+    p = _pair(W, H, D, seed)
+    gl, gr = p.gray()
+    assert np.array_equal(pm(S.ad_gray(gl, gr, D, L.RTS_L2R)), oracle.ad_gray(gl, gr, D, 1))
+    assert np.array_equal(pm(S.ad_gray(gl, gr, D, L.RTS_R2L)), oracle.ad_gray(gl, gr, D, -1))
+    fl, fr = (gl * np.float32(0.37)).astype(np.float32), (gr * np.float32(0.41)).astype(np.float32)
+    assert np.array_equal(pm(S.ad_float(fl, fr, D)), oracle.ad_float(fl, fr, D, 1))
+    assert np.array_equal(pm(S.ad_float(fl, fr, D, L.RTS_R2L)), oracle.ad_float(fl, fr, D, -1))
+    for trunc, norm in ((None, False), (None, True), (20.0, False), (40.0, True), (7.5, False)):
+        for direction, odir in ((L.RTS_L2R, 1), (L.RTS_R2L, -1)):
+            got = pm(S.ad_color(p.left, p.right, D, direction, trunc, norm))
+            assert np.array_equal(got, oracle.ad_color(p.left, p.right, D, odir, trunc, norm)), (trunc, norm, direction)
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES)
+@pytest.mark.parametrize("hr,wr", [(4, 3), (1, 1), (2, 6), (4, 7)])
+def test_census_hamming(W, H, D, seed, hr, wr):
+    p = _pair(W, H, D, seed)
+    gl, gr = p.gray()
+    bl, br = oracle.census(gl, hr, wr), oracle.census(gr, hr, wr)
+    assert np.array_equal(S.census_bools(gl, hr, wr), bl)
+    wl, wr_ = S.census(gl, hr, wr), S.census(gr, hr, wr)
+    assert np.array_equal(S.census_pack(bl), wl)
+    assert np.array_equal(pm(S.hamming(wl, wr_, D)), oracle.hamming(bl, br, hr, wr, D))
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
+def test_rank_log(W, H, D, seed):
+    p = _pair(W, H, D, seed)
+    gl, _ = p.gray()
+    for m in (1, 3, 7):
+        if m < min(W, H):
+            assert np.array_equal(S.rank(gl, m), oracle.rank(gl, m)), m
+    assert np.array_equal(S.convolve(gl, oracle.LOG5, 2, 2), oracle.convolve(gl, oracle.LOG5, 2, 2))
+    mask = np.linspace(-1, 1, 21).astype(np.float32)
+    assert np.array_equal(S.convolve(gl, mask, 1, 3), oracle.convolve(gl, mask, 1, 3))
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
+@pytest.mark.parametrize("hr,wr", [(1, 1), (3, 3), (2, 1)])
+def test_ncc(W, H, D, seed, hr, wr):
+    p = _pair(W, H, D, seed)
+    gl, gr = p.gray()
+    # Reference order: bit-exact, including the untouched flat-window entries.
+    init = np.full((D, H, W), -7.0, np.float32)
+    got = pm(S.ncc(gl, gr, D, hr, wr, L.RTS_NCC_SEQ | L.RTS_NCC_KEEP_FLAT, init=init))
+    want = oracle.ncc(gl, gr, D, hr, wr, literal=True, init=pm(init))
+    assert np.array_equal(got, want)
+    assert np.array_equal(pm(S.ncc(gl, gr, D, hr, wr, L.RTS_NCC_SEQ)), oracle.ncc(gl, gr, D, hr, wr))
+    # Integral form: exact integer sums, FP64 combine; vs the FP64 oracle.
+    f64 = oracle.ncc_f64(gl, gr, D, hr, wr)
+    got = pm(S.ncc(gl, gr, D, hr, wr, L.RTS_NCC_INTEGRAL)).astype(np.float64)
+    assert np.max(np.abs(got - f64) / np.maximum(np.abs(f64), 1e-30)) <= 1e-6
+
+
+def test_combine():
+    rng = np.random.default_rng(0)
+    c1 = rng.integers(0, 63, size=(5, 17, 33)).astype(np.float32)
+    c2 = (rng.random((5, 17, 33)) * 765).astype(np.float32)
+    got = S.combine(c1, c2, 30.0, 10.0).astype(np.float64)
+    f64 = oracle.combine_f64(c1, c2, 30.0, 10.0)
+    # relative to the FP64 oracle: <= 1e-5 (north-star float tolerance)
+    assert np.max(np.abs(got - f64) / np.maximum(f64, 1e-30)) <= 1e-5
+    ref32 = oracle.combine(c1, c2, 30.0, 10.0)
+    assert np.max(np.abs(got - ref32) / np.maximum(ref32, 1e-30)) <= 1e-5
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES)
+@pytest.mark.parametrize("r", [1, 4, 5, 17])
+def test_box(W, H, D, seed, r):
+    if r >= min(W, H):
+        pytest.skip("radius larger than the image")
+    p = _pair(W, H, D, seed)
+    c = oracle.ad_color(p.left, p.right, D, 1, 20.0, False)
+    assert np.array_equal(pm(S.box(dm(c), r)), oracle.box(c, r))
+    f = oracle.ad_float(oracle.convolve(p.gray()[0], oracle.LOG5, 2, 2), oracle.convolve(p.gray()[1], oracle.LOG5, 2, 2), D)
+    assert np.array_equal(pm(S.box(dm(f), r)), oracle.box(f, r))
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
+def test_aw_weights(W, H, D, seed):
+    p = _pair(W, H, D, seed)
+    ll, lr = p.lab()
+    r = 3
+    prox = S.prox_weight(36.0, r)
+    assert np.allclose(prox, oracle.prox_weight(36.0, r), rtol=2e-7, atol=0)
+    po = oracle.prox_weight(36.0, r)
+    wl, wr = S.compute_weights(ll, po, 7.0, r), S.compute_weights(lr, po, 7.0, r)
+    wlo, wro = oracle.compute_weights(ll, po, 7.0, r), oracle.compute_weights(lr, po, 7.0, r)
+    assert np.allclose(wl, wlo, rtol=3e-7, atol=1e-30)
+    c = oracle.ad_color(p.left, p.right, D, 1, 40.0, False)
+    got = pm(S.aw_weights(wlo, wro, dm(c), r))
+    assert np.array_equal(got, oracle.aw_l2r(wlo, wro, c, r))
+
+
+@pytest.mark.parametrize("W,H,D,seed,r", [s + (3,) for s in SIZES[:3]] + [s + (16,) for s in SIZES[:2]])
+def test_aw_on_the_fly(W, H, D, seed, r):
+    p = _pair(W, H, D, seed)
+    ll, lr = p.lab()
+    c = oracle.ad_color(p.left, p.right, D, 1, 40.0, False)
+    ys, xs = np.mgrid[0:H, 0:W]
+    f64 = oracle.aw_l2r_f64(ll, lr, c, ys.ravel(), xs.ravel(), r, 7.0, 36.0).reshape(H, W, D)
+    got32 = pm(S.aw(ll, lr, dm(c), r, 7.0, 36.0, L.RTS_AW_FP32)).astype(np.float64)
+    err32 = np.max(np.abs(got32 - f64) / np.maximum(np.abs(f64), 1e-30))
+    assert err32 <= 1e-5, err32
+    got16 = pm(S.aw(ll, lr, dm(c), r, 7.0, 36.0, L.RTS_AW_FP16)).astype(np.float64)
+    err16 = np.max(np.abs(got16 - f64))
+    print(f"AW r={r}: fp32 max rel err {err32:.2e}, fp16 max abs err {err16:.3f}")
+    d64 = np.argmin(f64, -1)
+    assert (np.argmin(got16, -1) != d64).mean() <= 0.01
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES)
+def test_cross(W, H, D, seed):
+    p = _pair(W, H, D, seed)
+    for Lm, tao in ((17, 20), (5, 12), (1, 255)):
+        arms, horz, vert = S.find_cross(p.left, Lm, tao)
+        ho, vo = oracle.find_cross(p.left, Lm, tao)
+        assert np.array_equal(horz, ho) and np.array_equal(vert, vo), (Lm, tao)
+        assert np.array_equal(S.pack_arms(ho, vo), arms)
+    gl, gr = p.gray()
+    c = oracle.combine(oracle.hamming(oracle.census(gl, 4, 3), oracle.census(gr, 4, 3), 4, 3, D),
+                       oracle.ad_color(p.left, p.right, D, 1), 30.0, 10.0)
+    P = oracle.hprefix(c)
+    assert np.array_equal(pm(S.hprefix(dm(c))), P)
+    al, hl, vl = S.find_cross(p.left)
+    ar, hr, vr = S.find_cross(p.right)
+    A = oracle.cross_v(P, hl, hr, vl, vr)
+    assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT)), A)
+    f64 = oracle.cross_f64(c, hl, hr, vl, vr)
+    fast = pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_FAST)).astype(np.float64)
+    assert np.max(np.abs(fast - f64) / np.maximum(f64, 1e-30)) <= 1e-5
+    fused = pm(S.cross_fused(dm(c), al, ar)).astype(np.float64)
+    assert np.max(np.abs(fused - f64) / np.maximum(f64, 1e-30)) <= 1e-5
+    # integer costs: every path is exact
+    ci = oracle.ad_color(p.left, p.right, D, 1, 24.0, False)
+    Ai = oracle.cross_v(oracle.hprefix(ci), hl, hr, vl, vr)
+    assert np.array_equal(pm(S.cross_v(dm(oracle.hprefix(ci)), al, ar, L.RTS_CROSS_FAST)), Ai)
+    assert np.array_equal(pm(S.cross_fused(dm(ci), al, ar)), Ai)
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES)
+def test_wta_lr(W, H, D, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, 6, size=(H, W, D)).astype(np.float32)   # ties everywhere
+    for maximize in (False, True):
+        flags = L.RTS_WTA_MAX if maximize else 0
+        dl, dr, m = S.wta(dm(v), flags, True, 1)
+        assert np.array_equal(dl, oracle.wta(v, maximize))
+        assert np.array_equal(dr, oracle.wta_r2l_diag(v, maximize))
+        assert np.array_equal(m, oracle.cross_check(dl, dr, 1))
+        dl2, dr2, m2 = S.wta(dm(v), flags | L.RTS_WTA_LITERAL, True, 0)
+        assert np.array_equal(dl2, oracle.wta(v, maximize, literal=True))
+        assert np.array_equal(m2, oracle.cross_check(dl2, dr2, 0))
+    dl, _, _ = S.wta(dm(v), 0, False)
+    assert np.array_equal(dl, oracle.wta(v))
+    assert np.array_equal(S.cross_check(dl, dr, 2), oracle.cross_check(dl, dr, 2))
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
+def test_refine(W, H, D, seed):
+    p = _pair(W, H, D, seed)
+    arms, horz, vert = S.find_cross(p.left)
+    rng = np.random.default_rng(seed)
+    disp = rng.integers(0, D, size=(H, W)).astype(np.uint8)
+    assert np.array_equal(S.disp_refine(disp, arms, D), oracle.disp_refine(disp, horz, vert, D))
+    assert np.array_equal(S.disp_refine(disp, arms, D, literal=True), oracle.disp_refine(disp, horz, vert, D, literal=True))
+
+
+def test_layout_roundtrip_and_gray():
+    rng = np.random.default_rng(1)
+    v = rng.random((7, 13, 29)).astype(np.float32)
+    from rtstereo_amd.device import DeviceArray
+    from rtstereo_amd._lib import call
+    a = DeviceArray.from_host(pm(v))
+    b = DeviceArray((7, 13, 29), np.float32)
+    call("rts_vol_to_dm", a.ptr, b.ptr, 29, 13, 7, None)
+    assert np.array_equal(b.download(), v)
+    c = DeviceArray((13, 29, 7), np.float32)
+    call("rts_vol_to_pm", b.ptr, c.ptr, 29, 13, 7, None)
+    assert np.array_equal(c.download(), pm(v))
+    p = _pair(33, 17, 4, 3)
+    assert np.array_equal(S.rgb_to_gray(p.left), gen.to_gray(p.left))
+
+
+def test_errors_are_reported():
+    from rtstereo_amd._lib import RtsError
+    with pytest.raises(RtsError):
+        S.ad_gray(np.zeros((4, 4), np.uint8), np.zeros((4, 4), np.uint8), 300)   # D > 256
+    with pytest.raises(RtsError):
+        S.box(np.zeros((2, 4, 4), np.float32), 9)                                # radius > image

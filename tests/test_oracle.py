@@ -1,0 +1,385 @@
+"""Pins the CPU oracle (oracle/rts_oracle.c).
+
+1. Against the golden fixtures written by the reference's own seq/ stage
+   functions (tests/golden/make_golden.py): bit-exact in the literal modes.
+2. Against the reference library itself (oracle/_ref), when it was built
+   here, on further seeded sizes.
+3. Against what the mathematics fixes, independently of the reference:
+   numpy restatements by other routes (cumulative sums, popcounts, brute
+   force), closed forms and invariants -- chosen so that a dropped term, a
+   wrong sign or index, or a transposed operand fails one of them.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from rtstereo_amd import gen
+from conftest import load_golden, golden_files
+
+
+# ---------------------------------------------------------------- 1. golden vectors
+
+def test_golden_ad(golden):
+    g = golden
+    L, R, gl, gr, D = g["left"], g["right"], g["gray_l"], g["gray_r"], int(g["D"])
+    assert np.array_equal(oracle.ad_gray(gl, gr, D, 1), g["ad_gray_l2r"])
+    assert np.array_equal(oracle.ad_gray(gl, gr, D, -1), g["ad_gray_r2l"])
+    assert np.array_equal(oracle.ad_float(g["float_l"], g["float_r"], D, 1), g["ad_float_l2r"])
+    assert np.array_equal(oracle.ad_float(g["float_l"], g["float_r"], D, -1), g["ad_float_r2l"])
+    assert np.array_equal(oracle.ad_color(L, R, D, 1), g["ad_color"])
+    assert np.array_equal(oracle.ad_color(L, R, D, 1, norm=True), g["ad_color_norm"])
+    assert np.array_equal(oracle.ad_color(L, R, D, 1, trunc=20.0), g["ad_color_t20"])
+    assert np.array_equal(oracle.ad_color(L, R, D, 1, trunc=20.0, norm=True), g["ad_color_t20_norm"])
+    assert np.array_equal(oracle.ad_color(L, R, D, -1), g["ad_color_r2l"])
+    assert np.array_equal(oracle.ad_color(L, R, D, -1, trunc=24.0), g["ad_color_r2l_t24"])
+
+
+def test_golden_census_rank_log(golden):
+    g = golden
+    D = int(g["D"])
+    assert np.array_equal(oracle.census(g["gray_l"], 4, 3), g["census_l"])
+    assert np.array_equal(oracle.census(g["gray_r"], 4, 3), g["census_r"])
+    assert np.array_equal(oracle.hamming(g["census_l"], g["census_r"], 4, 3, D), g["hamming"])
+    assert np.array_equal(oracle.rank(g["gray_l"], 3), g["rank3_l"])
+    assert np.array_equal(oracle.rank(g["gray_l"], 7), g["rank7_l"])
+    assert np.array_equal(oracle.convolve(g["gray_l"], oracle.LOG5, 2, 2), g["log_l"])
+    assert np.array_equal(oracle.convolve(g["gray_r"], oracle.LOG5, 2, 2), g["log_r"])
+
+
+def test_golden_ncc_combine_box(golden):
+    g = golden
+    D = int(g["D"])
+    H, W = g["gray_l"].shape
+    init = np.full((H, W, D), -7.0, np.float32)
+    assert np.array_equal(oracle.ncc(g["gray_l"], g["gray_r"], D, 1, 1, literal=True, init=init), g["ncc_r1"])
+    assert np.array_equal(oracle.ncc(g["gray_l"], g["gray_r"], D, 3, 3, literal=True, init=init), g["ncc_r3"])
+    assert np.array_equal(oracle.combine(g["hamming"], g["ad_color"], 30.0, 10.0), g["combine"])
+    assert np.array_equal(oracle.box(g["ad_color_t20"], 2, literal=True), g["box_r2_inplace"])
+
+
+def test_golden_aw(golden):
+    g = golden
+    prox = oracle.prox_weight(36.0, 3)
+    assert np.array_equal(prox, g["prox_r3"])
+    wl = oracle.compute_weights(g["lab_l"], prox, 7.0, 3)
+    wr = oracle.compute_weights(g["lab_r"], prox, 7.0, 3)
+    assert np.array_equal(wl[0], g["weights_l_r3_row0"])
+    assert np.array_equal(oracle.aw_l2r(wl, wr, g["ad_color_t20"], 3), g["aw_r3"])
+
+
+def test_golden_cross_wta(golden):
+    g = golden
+    D = int(g["D"])
+    hl, vl = oracle.find_cross(g["left"])
+    hr, vr = oracle.find_cross(g["right"])
+    for a, k in ((hl, "horz_l"), (vl, "vert_l"), (hr, "horz_r"), (vr, "vert_r")):
+        assert np.array_equal(a, g[k]), k
+    P = oracle.hprefix(g["combine"])
+    assert np.array_equal(P, g["hprefix"])
+    assert np.array_equal(oracle.cross_v(P, hl, hr, vl, vr), g["cross"])
+    assert np.array_equal(oracle.wta(g["cross"], literal=True), g["wta"])
+    assert np.array_equal(oracle.wta(g["ncc_r1"], maximize=True, literal=True), g["wta_ncc"])
+    assert np.array_equal(oracle.wta(g["box_r2_inplace"], literal=True), g["wta_box"])
+    assert np.array_equal(oracle.cross_check(g["wta"], g["dr_diag"]), g["cross_check"])
+    assert np.array_equal(oracle.disp_refine(g["wta"], hl, vl, D, literal=True), g["refine"])
+
+
+def test_golden_plain_full():
+    g = load_golden(golden_files("seq_plain_384x288.npz")[0])
+    p = gen.plain_384x288()
+    assert p.digest() == str(g["digest"])
+    cost = oracle.ad_color(p.left, p.right, 16, 1, trunc=20.0)
+    disp = oracle.wta(oracle.box(cost, 4, literal=True), literal=True)
+    assert np.array_equal(disp, g["disp_literal"])
+
+
+def test_golden_census_cross_pipeline():
+    g = load_golden(golden_files("seq_census_cross_160x96.npz")[0])
+    p = gen.make_pair(160, 96, 24, 10, 3.0, seed=11)
+    assert p.digest() == str(g["digest"])
+    gl, gr = p.gray()
+    c = oracle.combine(oracle.hamming(oracle.census(gl, 4, 3), oracle.census(gr, 4, 3), 4, 3, 24),
+                       oracle.ad_color(p.left, p.right, 24, 1), 30.0, 10.0)
+    hl, vl = oracle.find_cross(p.left)
+    hr, vr = oracle.find_cross(p.right)
+    A = oracle.cross_v(oracle.hprefix(c), hl, hr, vl, vr)
+    import hashlib
+    assert hashlib.sha256(A.tobytes()).hexdigest() == str(g["vol_sha"])
+    assert np.array_equal(oracle.wta(A, literal=True), g["disp_literal"])
+
+
+# ---------------------------------------------------------------- 2. the reference itself
+
+ref = oracle.ref_lib()
+needs_ref = pytest.mark.skipif(ref is None, reason="reference seq/ sources not built here")
+
+
+@needs_ref
+@pytest.mark.parametrize("W,H,D,seed", [(33, 19, 7, 1), (64, 40, 16, 2), (18, 12, 18, 3)])
+def test_ref_stages(W, H, D, seed):
+    p = gen.make_pair(W, H, D, 3, 2.5, seed)
+    L, R = p.left, p.right
+    gl, gr = p.gray()
+    assert np.array_equal(oracle.ad_color(L, R, D, 1, trunc=7.5, norm=True),
+                          ref.ad("pixelBasedCostL2R_Color_trunc", L, R, D, 7.5, 1))
+    c1 = ref.hamming(ref.census(gl, 2, 5), ref.census(gr, 2, 5), 2, 5, D)
+    assert np.array_equal(oracle.hamming(oracle.census(gl, 2, 5), oracle.census(gr, 2, 5), 2, 5, D), c1)
+    c2 = ref.ad("pixelBasedCostL2R_Color", L, R, D, 0)
+    cm = ref.combine(c1, c2, 13.0, 7.0)
+    assert np.array_equal(oracle.combine(c1, c2, 13.0, 7.0), cm)
+    assert np.array_equal(oracle.box(cm, 3, literal=True), ref.box(cm, 3))
+    hl, vl = ref.find_cross(L)
+    hr, vr = ref.find_cross(R)
+    P = ref.hprefix(cm)
+    A = ref.cross_v(P, hl, hr, vl, vr)
+    assert np.array_equal(oracle.cross_v(oracle.hprefix(cm), hl, hr, vl, vr), A)
+    assert np.array_equal(oracle.wta(A, literal=True), ref.wta(A))
+    assert np.array_equal(oracle.ncc(gl, gr, D, 2, 1, literal=True), ref.ncc(gl, gr, D, 2, 1))
+
+
+# ---------------------------------------------------------------- 3. mathematics
+
+def _pair(W=37, H=23, D=9, seed=5):
+    return gen.make_pair(W, H, D, 3, 2.0, seed)
+
+
+def test_math_ad_closed_form():
+    p = _pair()
+    L, R = p.left.astype(int), p.right.astype(int)
+    H, W, _ = L.shape
+    D = 9
+    want = np.zeros((H, W, D), np.float32)
+    for d in range(D):
+        Rs = np.roll(R, d, axis=1)     # Rs[:, x] = R[:, x-d wrapped]
+        want[:, :, d] = np.minimum(np.abs(L - Rs).sum(-1), 20)
+    assert np.array_equal(oracle.ad_color(p.left, p.right, D, 1, trunc=20.0), want)
+    want_r = np.stack([np.abs(L - np.roll(R, -d, axis=1)).sum(-1) for d in range(D)], -1).astype(np.float32)
+    assert np.array_equal(oracle.ad_color(p.left, p.right, D, -1), want_r)
+
+
+def test_math_census_hamming_popcount():
+    p = _pair()
+    gl, gr = p.gray()
+    b = oracle.census(gl, 4, 3)
+    # brute force with numpy rolls: entry k=(dy+4)*7+(dx+3) is centre > neighbour
+    g = gl.astype(int)
+    k = 0
+    for dy in range(-4, 5):
+        for dx in range(-3, 4):
+            nb = np.roll(np.roll(g, -dy, 0), -dx, 1)
+            assert np.array_equal(b[:, :, k], (g > nb).astype(np.uint8)), (dy, dx)
+            k += 1
+    assert not b[:, :, 31].any()           # the centre entry is never set
+    br = oracle.census(gr, 4, 3)
+    bits = np.packbits(b, axis=-1, bitorder="little").view(np.uint64)[..., 0]
+    bitsr = np.packbits(br, axis=-1, bitorder="little").view(np.uint64)[..., 0]
+    h = oracle.hamming(b, br, 4, 3, 9)
+    for d in range(9):
+        x = bits ^ np.roll(bitsr, d, axis=1)
+        pc = np.array([bin(int(v)).count("1") for v in x.ravel()]).reshape(x.shape)
+        assert np.array_equal(h[:, :, d], pc.astype(np.float32))
+    assert h.max() <= 62
+
+
+def test_math_rank_bounds_and_brute():
+    p = _pair()
+    g, _ = p.gray()
+    r = oracle.rank(g, 2).astype(int)
+    gi = g.astype(int)
+    want = np.ones_like(gi)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            want += np.roll(np.roll(gi, -dy, 0), -dx, 1) < gi
+    assert np.array_equal(r, want)
+    assert r.min() >= 1 and r.max() <= 25
+
+
+def test_math_box_is_wrapped_uniform_mean():
+    rng = np.random.default_rng(3)
+    c = rng.integers(0, 60, size=(15, 22, 4)).astype(np.float32)
+    r = 2
+    out = oracle.box(c, r)
+    # integer sums via 2-D cumulative sums of the wrapped padding: exact
+    pad = np.pad(c.astype(np.int64), ((r, r), (r, r), (0, 0)), mode="wrap")
+    cs = np.zeros((pad.shape[0] + 1, pad.shape[1] + 1, 4), np.int64)
+    cs[1:, 1:] = pad.cumsum(0).cumsum(1)
+    n = 2 * r + 1
+    s = cs[n:, n:] - cs[:-n, n:] - cs[n:, :-n] + cs[:-n, :-n]
+    assert np.array_equal(out, (s.astype(np.float32) / np.float32(n * n)))
+    # constant volume is unchanged; literal (in-place) differs from a box
+    assert np.array_equal(oracle.box(np.full((9, 9, 2), 5.0, np.float32), 2), np.full((9, 9, 2), 5.0, np.float32))
+    assert not np.array_equal(oracle.box(c, r, literal=True), out)
+
+
+def test_math_log_matches_correlation():
+    p = _pair()
+    g, _ = p.gray()
+    f64 = oracle.convolve(g, oracle.LOG5, 2, 2, f64=True)
+    m = oracle.LOG5.astype(np.float64).reshape(5, 5)
+    gi = g.astype(np.float64)
+    acc = np.zeros_like(gi)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            acc += m[dy + 2, dx + 2] * np.roll(np.roll(gi, -dy, 0), -dx, 1)
+    want = np.minimum(np.abs(acc), 255)
+    assert np.allclose(f64, want, rtol=0, atol=1e-9)
+    f32 = oracle.convolve(g, oracle.LOG5, 2, 2)
+    assert np.allclose(f32, want, rtol=1e-5, atol=1e-4)
+
+
+def test_math_ncc_formula():
+    p = _pair(W=21, H=13, D=5)
+    gl, gr = p.gray()
+    hr, wr, D = 1, 2, 5
+    got = oracle.ncc_f64(gl, gr, D, hr, wr)
+    a = gl.astype(np.float64)
+    H, W = a.shape
+    for d in range(D):
+        b = np.roll(gr.astype(np.float64), d, axis=1)
+        wins_a = np.stack([np.roll(np.roll(a, -dy, 0), -dx, 1) for dy in range(-hr, hr + 1) for dx in range(-wr, wr + 1)])
+        wins_b = np.stack([np.roll(np.roll(b, -dy, 0), -dx, 1) for dy in range(-hr, hr + 1) for dx in range(-wr, wr + 1)])
+        ac = wins_a - wins_a.mean(0)
+        bc = wins_b - wins_b.mean(0)
+        nom = (ac * bc).sum(0)
+        den = np.sqrt((ac * ac * bc * bc).sum(0))
+        want = np.where(den != 0, nom / np.where(den == 0, 1, den), 0)
+        assert np.allclose(got[:, :, d], want, rtol=1e-12, atol=1e-12)
+    f32 = oracle.ncc(gl, gr, D, hr, wr)
+    assert np.allclose(f32, got, rtol=1e-4, atol=1e-4)
+
+
+def test_math_combine():
+    rng = np.random.default_rng(1)
+    c1 = rng.integers(0, 63, size=(4, 5, 6)).astype(np.float32)
+    c2 = rng.integers(0, 766, size=(4, 5, 6)).astype(np.float32)
+    want = (1 - np.exp(-c1.astype(np.float64) / 30)) + (1 - np.exp(-c2.astype(np.float64) / 10))
+    assert np.allclose(oracle.combine_f64(c1, c2, 30.0, 10.0), want, rtol=1e-15)
+    assert np.allclose(oracle.combine(c1, c2, 30.0, 10.0), want, rtol=1e-6)
+    got = oracle.combine(c1, c2, 30.0, 10.0)
+    assert got.min() >= 0 and got.max() < 2
+
+
+def _arm_brute(img, y, x, sy, sx, L, tao):
+    H, W, _ = img.shape
+    n = 0
+    while n < L:
+        yy, xx = y + sy * (n + 1), x + sx * (n + 1)
+        if not (0 <= yy < H and 0 <= xx < W):
+            break
+        n += 1
+        if np.abs(img[yy, xx].astype(int) - img[y, x].astype(int)).max() >= tao:
+            break
+    return n
+
+
+def test_math_cross_arms_brute():
+    p = _pair(W=30, H=20)
+    horz, vert = oracle.find_cross(p.left, L=6, tao=12)
+    for y in range(20):
+        for x in range(30):
+            assert horz[y, x, 0] == _arm_brute(p.left, y, x, 0, -1, 6, 12)
+            assert horz[y, x, 1] == _arm_brute(p.left, y, x, 0, 1, 6, 12)
+            assert vert[y, x, 0] == _arm_brute(p.left, y, x, -1, 0, 6, 12)
+            assert vert[y, x, 1] == _arm_brute(p.left, y, x, 1, 0, 6, 12)
+
+
+def test_math_cross_aggregation():
+    p = _pair(W=26, H=17, D=4)
+    rng = np.random.default_rng(9)
+    c = rng.integers(0, 40, size=(17, 26, 4)).astype(np.float32)
+    hl, vl = oracle.find_cross(p.left, L=5, tao=15)
+    hr, vr = oracle.find_cross(p.right, L=5, tao=15)
+    got = oracle.cross_v(oracle.hprefix(c), hl, hr, vl, vr)
+    f64 = oracle.cross_f64(c, hl, hr, vl, vr)
+    # integer costs: every float sum is exact, so the float path equals the
+    # exact support mean (including the column-0 drop)
+    assert np.array_equal(got, f64.astype(np.float32))
+    # brute force of the support region
+    for (y, x, d) in [(0, 0, 0), (5, 7, 2), (16, 25, 3), (8, 2, 3), (3, 0, 1)]:
+        tx = (x - d) % 26
+        t = min(vl[y, x, 0], vr[y, tx, 0])
+        b = min(vl[y, x, 1], vr[y, tx, 1])
+        s = 0.0
+        n = 0
+        for yy in range(y - t, y + b + 1):
+            l = min(hl[yy, x, 0], hr[yy, tx, 0])
+            r = min(hl[yy, x, 1], hr[yy, tx, 1])
+            lo = 1 if x - l - 1 < 0 else x - l
+            s += c[yy, lo:x + r + 1, d].sum()
+            n += l + r + 1
+        assert got[y, x, d] == np.float32(s / n)
+    # invariants: a constant volume stays constant away from column 0
+    k = np.full((17, 26, 4), 3.0, np.float32)
+    out = oracle.cross_f64(k, hl, hr, vl, vr)
+    assert np.all(out <= 3.0 + 1e-12) and np.all(out[:, 18:, :] == 3.0)
+
+
+def test_math_wta_and_check():
+    rng = np.random.default_rng(2)
+    v = rng.integers(0, 5, size=(6, 11, 7)).astype(np.float32)   # many ties
+    dl = oracle.wta(v)
+    assert np.array_equal(dl, np.argmin(v, axis=-1).astype(np.uint8))     # lowest d on ties
+    assert np.array_equal(oracle.wta(v, maximize=True), np.argmax(v, axis=-1).astype(np.uint8))
+    dr = oracle.wta_r2l_diag(v)
+    H, W, D = v.shape
+    diag = np.stack([np.roll(v[:, :, d], -d, axis=1) for d in range(D)], -1)   # v[(x+d)%W, d]
+    assert np.array_equal(dr, np.argmin(diag, -1).astype(np.uint8))
+    m = oracle.cross_check(dl, dr, 1)
+    xs = (np.arange(W)[None, :] - dl.astype(int)) % W
+    want = np.abs(dl.astype(int) - np.take_along_axis(dr.astype(int), xs, 1)) <= 1
+    assert np.array_equal(m, want.astype(np.uint8))
+    # literal carry: a pixel where d=0 is not beaten repeats the previous pixel
+    lit = oracle.wta(v, literal=True).ravel()
+    flat = v.reshape(-1, D)
+    beaten = (flat[:, 1:].min(1) < flat[:, 0])
+    prev = 0
+    for i in range(flat.shape[0]):
+        exp = int(np.argmin(flat[i])) if beaten[i] else prev
+        assert lit[i] == exp
+        prev = exp
+
+
+def test_math_aw_uniform_and_limit():
+    p = _pair(W=19, H=14, D=3)
+    lab_l, lab_r = p.lab()
+    r = 2
+    prox = oracle.prox_weight(36.0, r)
+    e = 2 * r + 1
+    want_prox = np.exp(-np.sqrt((np.arange(e)[:, None] - r) ** 2 + (np.arange(e)[None, :] - r) ** 2) / 36.0)
+    assert np.allclose(prox, want_prox.ravel().astype(np.float32), rtol=1e-6)
+    wl = oracle.compute_weights(lab_l, prox, 7.0, r)
+    wr = oracle.compute_weights(lab_r, prox, 7.0, r)
+    k = np.full((14, 19, 3), 4.0, np.float32)
+    assert np.allclose(oracle.aw_l2r(wl, wr, k, r), 4.0, rtol=1e-6)       # constant stays constant
+    rng = np.random.default_rng(4)
+    c = rng.integers(0, 40, size=(14, 19, 3)).astype(np.float32)
+    ys, xs = np.mgrid[0:14, 0:19]
+    f64 = oracle.aw_l2r_f64(lab_l, lab_r, c, ys.ravel(), xs.ravel(), r, 7.0, 36.0).reshape(14, 19, 3)
+    f32 = oracle.aw_l2r(wl, wr, c, r)
+    assert np.allclose(f32, f64, rtol=2e-5)
+    # gamma_c -> infinity and gamma_p -> infinity: weights 1 inside the image
+    # (0 outside), so the result is the plain mean over in-image positions
+    big = oracle.aw_l2r_f64(lab_l, lab_r, c, [7], [9], r, 1e30, 1e30)
+    win = c[5:10, 7:12, :].astype(np.float64).reshape(-1, 3).mean(0)
+    assert np.allclose(big[0], win, rtol=1e-12)
+
+
+def test_math_refine_vote():
+    rng = np.random.default_rng(6)
+    disp = rng.integers(0, 4, size=(9, 12)).astype(np.uint8)
+    horz = np.full((9, 12, 2), 1, np.int32)
+    vert = np.full((9, 12, 2), 1, np.int32)
+    horz[:, 0, 0] = 0
+    horz[:, -1, 1] = 0
+    vert[0, :, 0] = 0
+    vert[-1, :, 1] = 0
+    out = oracle.disp_refine(disp, horz, vert, 4)
+    for y in range(9):
+        for x in range(12):
+            rows = range(y - vert[y, x, 0], y + vert[y, x, 1])      # exclusive end
+            votes = np.zeros(4, int)
+            for yy in rows:
+                lo, hi = max(x - horz[yy, x, 0], 0), min(x + horz[yy, x, 1], 11)
+                for v in disp[yy, lo:hi + 1]:
+                    votes[v] += 1
+            assert out[y, x] == int(np.argmax(votes))
