@@ -271,6 +271,12 @@ rts_status rts_pipeline_destroy(rts_pipeline *p);
 rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
                             const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
                             const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask, void *stream);
+/* Per-stage timing: when enabled, each run records HIP events on its stream
+ * between its stages ("prep": gray/census/arms, "cost": matching cost,
+ * "aggr": aggregation, "wta": disparity selection); rts_pipeline_stage_ms
+ * waits for the last run and returns the stage durations in ms. */
+rts_status rts_pipeline_enable_timing(rts_pipeline *p, int on);
+rts_status rts_pipeline_stage_ms(rts_pipeline *p, float *ms, const char **names, int max, int *n);
 /* Device pointer of the final aggregated float volume of the last run
  * (disparity-major), for inspection. */
 const float *rts_pipeline_volume(rts_pipeline *p);

@@ -247,7 +247,22 @@ void ora_ncc_f64(const uint8_t *ref, const uint8_t *tar, double *cost, int D, in
                 for (int dy = -hr; dy <= hr; dy++)
                     for (int dx = -wr; dx <= wr; dx++)
                         mt += tar[IDX(wrap1(y + dy, H), wrap1(x + dx - d, W), W)];
+                /* The denominator is exactly 0 iff every tap has a centred ref
+                 * or tar value of 0: decide that in integers, since the
+                 * rounded double sum can leave a tiny non-zero residue. */
+                const long sa = (long)(mr * area + 0.5), sb = (long)(mt + 0.5);
+                int flat = 1;
+                for (int dy = -hr; dy <= hr && flat; dy++)
+                    for (int dx = -wr; dx <= wr; dx++) {
+                        const long a = (long)area * ref[IDX(wrap1(y + dy, H), wrap1(x + dx, W), W)] - sa;
+                        const long b = (long)area * tar[IDX(wrap1(y + dy, H), wrap1(x + dx - d, W), W)] - sb;
+                        if (a != 0 && b != 0) { flat = 0; break; }
+                    }
                 mt /= area;
+                if (flat) {
+                    cost[d + (size_t)D * p] = 0.0;
+                    continue;
+                }
                 double nom = 0.0, den = 0.0;
                 for (int dy = -hr; dy <= hr; dy++)
                     for (int dx = -wr; dx <= wr; dx++) {

@@ -63,13 +63,13 @@ __global__ void k_prox(float *prox, float gp, int r) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= e * e) return;
     const int dy = k / e - r, dx = k % e - r;
-    prox[k] = expf(__fdiv_rn(-__fsqrt_rn((float)(dy * dy + dx * dx)), gp));
+    prox[k] = expf(__fdiv_rn(-sqrt_rn((float)(dy * dy + dx * dx)), gp));
 }
 
 // Lab distance in the reference's float order (seq/AdaptiveWeightAggregation.cpp:19-24).
 __device__ __forceinline__ float lab_dist(const uint8_t *a, const uint8_t *b) {
     const float e0 = (float)a[0] - (float)b[0], e1 = (float)a[1] - (float)b[1], e2 = (float)a[2] - (float)b[2];
-    return __fsqrt_rn(e0 * e0 + e1 * e1 + e2 * e2);
+    return sqrt_rn(e0 * e0 + e1 * e1 + e2 * e2);
 }
 
 // seq/AdaptiveWeightAggregation.cpp:33-71: w[p][k] = prox[k]*exp(-dist/gc),

@@ -39,7 +39,7 @@ struct AwLds {
 
 __device__ __forceinline__ float lab_dist3(const uint8_t *a, const uint8_t *b) {
     const float e0 = (float)a[0] - (float)b[0], e1 = (float)a[1] - (float)b[1], e2 = (float)a[2] - (float)b[2];
-    return __fsqrt_rn(e0 * e0 + e1 * e1 + e2 * e2);
+    return sqrt_rn(e0 * e0 + e1 * e1 + e2 * e2);
 }
 
 template <bool HALF>

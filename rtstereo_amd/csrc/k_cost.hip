@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_ncc_seq(const uint8_t *__restrict__ ref
                 den += ((a * a) * b) * b;
             }
         }
-        den = __fsqrt_rn(den);
+        den = sqrt_rn(den);
         const size_t o = d * pl + (size_t)y * W + x;
         if (den != 0.0f)
             vol[o] = __fdiv_rn(nom, den);

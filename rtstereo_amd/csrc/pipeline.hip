@@ -133,7 +133,20 @@ struct rts_pipeline {
     int nA = 0;
     int census_words = 1;
     bool fused_front = false;
+    // per-stage timing (HIP events recorded on the run's stream)
+    bool timing = false;
+    hipEvent_t ev[8] = {};
+    const char *mark_names[8] = {};
+    int nmarks = 0;
 };
+
+#define RTS_MARK(p, name, s)                                    \
+    do {                                                        \
+        if ((p)->timing && (p)->nmarks < 8) {                   \
+            RTS_HIP(hipEventRecord((p)->ev[(p)->nmarks], (s)));  \
+            (p)->mark_names[(p)->nmarks++] = (name);            \
+        }                                                       \
+    } while (0)
 
 namespace {
 
@@ -286,6 +299,8 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
     }
     (void)hipFree(p->tabC);
     (void)hipFree(p->tabA);
+    for (int i = 0; i < 8; i++)
+        if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
     delete p;
     return RTS_OK;
 }
@@ -345,6 +360,8 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
             if (int e = rts_rgb_to_gray(rgb[v], p->gray[v], W, H, s)) return e;
             gray[v] = p->gray[v];
         }
+    p->nmarks = 0;
+    RTS_MARK(p, "start", s);
     float *cur = p->vol[0], *other = p->vol[1];
     if (d.aggr == RTS_AGGR_CROSS)
         for (int v = 0; v < 2; v++)
@@ -355,6 +372,7 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
         if (census)
             for (int v = 0; v < 2; v++)
                 if (int e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, s)) return e;
+        RTS_MARK(p, "prep", s);
         const int waves = div_up(D, 64);
         const size_t shm = (size_t)W * ((census ? 8 : 0) + (ad ? 4 : 0)) + sizeof(float) * (65 + p->nA) +
                            sizeof(float) * waves * 64 * (FH_CH + 1);
@@ -372,12 +390,14 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
         RTS_LAUNCHED();
         prefixed = true;
     } else {
+        RTS_MARK(p, "prep", s);
         if (int e = build_cost(p, d.cost1, rgb, gray, cur, s)) return e;
         if (d.cost2) {
             if (int e = build_cost(p, d.cost2, rgb, gray, other, s)) return e;
             if (int e = rts_combine(cur, other, d.lambda1, d.lambda2, (size_t)W * H * D, s)) return e;
         }
     }
+    RTS_MARK(p, "cost", s);
     switch (d.aggr) {
         case RTS_AGGR_NONE:
             break;
@@ -393,17 +413,50 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
             break;
         case RTS_AGGR_CROSS:
             for (int it = 0; it < d.cross_iters; it++) {
-                if (!prefixed)
+                if (prefixed) {
+                    if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D,
+                                            d.cross_exact ? RTS_CROSS_EXACT : RTS_CROSS_FAST, s))
+                        return e;
+                } else if (d.cross_exact || d.cross_L > 17) {
                     if (int e = rts_hprefix(cur, W, H, D, s)) return e;
+                    if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D,
+                                            d.cross_exact ? RTS_CROSS_EXACT : RTS_CROSS_FAST, s))
+                        return e;
+                } else {
+                    // one kernel per pass: local row prefix, segments, column sums
+                    if (int e = rts_cross_fused(cur, other, p->arms[0], p->arms[1], W, H, D, s)) return e;
+                }
                 prefixed = false;
-                if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D,
-                                        d.cross_exact ? RTS_CROSS_EXACT : RTS_CROSS_FAST, s))
-                    return e;
                 std::swap(cur, other);
             }
             break;
     }
+    RTS_MARK(p, "aggr", s);
     p->final_vol = cur;
     int flags = (d.cost1 == RTS_COST_NCC ? RTS_WTA_MAX : 0) | (d.wta_literal ? RTS_WTA_LITERAL : 0);
-    return rts_wta(cur, dl, d.lr_check ? dr : nullptr, d.lr_check ? mask : nullptr, W, H, D, flags, d.lr_tol, s);
+    if (int e = rts_wta(cur, dl, d.lr_check ? dr : nullptr, d.lr_check ? mask : nullptr, W, H, D, flags, d.lr_tol, s))
+        return e;
+    RTS_MARK(p, "wta", s);
+    return RTS_OK;
+}
+
+extern "C" rts_status rts_pipeline_enable_timing(rts_pipeline *p, int on) {
+    RTS_CHECK(p != nullptr, "null pipeline");
+    if (on && !p->ev[0])
+        for (int i = 0; i < 8; i++) RTS_HIP(hipEventCreate(&p->ev[i]));
+    p->timing = on != 0;
+    return RTS_OK;
+}
+
+extern "C" rts_status rts_pipeline_stage_ms(rts_pipeline *p, float *ms, const char **names, int max, int *n) {
+    RTS_CHECK(p != nullptr && n != nullptr, "null argument");
+    *n = 0;
+    if (!p->timing || p->nmarks < 2) return RTS_OK;
+    RTS_HIP(hipEventSynchronize(p->ev[p->nmarks - 1]));
+    for (int i = 1; i < p->nmarks && *n < max; i++) {
+        RTS_HIP(hipEventElapsedTime(&ms[*n], p->ev[i - 1], p->ev[i]));
+        if (names) names[*n] = p->mark_names[i];
+        (*n)++;
+    }
+    return RTS_OK;
 }

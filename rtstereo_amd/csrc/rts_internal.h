@@ -49,6 +49,11 @@ __device__ __forceinline__ int wrapn(int v, int n) {
     return v < 0 ? v + n : v;
 }
 
+// Correctly rounded float square root.  The hardware v_sqrt_f32 (which
+// __fsqrt_rn lowers to here) is not correctly rounded; a double square root
+// rounded once to float is, because 53 >= 2*24 + 2 bits.
+__device__ __forceinline__ float sqrt_rn(float x) { return (float)__builtin_sqrt((double)x); }
+
 // Plane stride of a disparity-major volume [D][H][W].
 __host__ __device__ __forceinline__ size_t plane(int W, int H) { return (size_t)W * (size_t)H; }
 

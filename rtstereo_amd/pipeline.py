@@ -81,6 +81,18 @@ class Pipeline:
              right_lab.ptr if right_lab is not None else None,
              self.dl.ptr, self.dr.ptr, self.mask.ptr, stream)
 
+    def enable_timing(self, on: bool = True) -> None:
+        call("rts_pipeline_enable_timing", self.handle, int(on))
+
+    def stage_ms(self) -> dict:
+        """Stage durations (ms) of the last run, measured with HIP events on
+        the run's stream; empty unless timing is enabled."""
+        ms = (C.c_float * 8)()
+        names = (C.c_char_p * 8)()
+        n = C.c_int(0)
+        call("rts_pipeline_stage_ms", self.handle, ms, names, 8, C.byref(n))
+        return {names[i].decode(): float(ms[i]) for i in range(n.value)}
+
     def volume(self) -> np.ndarray:
         """Final aggregated volume of the last run, [D][H][W]."""
         d = self.desc

@@ -66,7 +66,7 @@ def test_pipeline_fast_cross_tolerance():
     hr, vr = oracle.find_cross(p.right)
     f64 = oracle.cross_f64(c, hl, hr, vl, vr)
     got = np.moveaxis(vol, 0, -1).astype(np.float64)
-    assert np.max(np.abs(got - f64) / f64) <= 1e-5
+    assert np.max(np.abs(got - f64) / np.maximum(f64, 1e-30)) <= 1e-5
     assert (dl != np.argmin(f64, -1)).mean() <= 1e-3
 
 
