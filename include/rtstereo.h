@@ -183,17 +183,21 @@ rts_status rts_arms_pack(const int *horz, const int *vert, uint32_t *arms, int W
  * seq/CrossBasedAggregation.cpp:117-125 aggregate_cost_horizontal. */
 rts_status rts_hprefix(float *vol, int W, int H, int D, void *stream);
 
-/* Vertical sweep of horizontal segments over the prefix volume P.
- * seq/CrossBasedAggregation.cpp:127-169 cross_stereo_aggregation.
- * RTS_CROSS_EXACT: float sums in the reference's row order (bit-exact);
- * RTS_CROSS_FAST: segment sums prefix-summed down each column in FP64. */
+/* Vertical sweep of horizontal segments over the prefix volume P: segment
+ * sums P[x+r] - P[max(x-l-1,0)] added in the reference's row order, then
+ * divided by the count -- bit-identical to
+ * seq/CrossBasedAggregation.cpp:127-169 cross_stereo_aggregation.  Both mode
+ * values run the same kernel (RTS_CROSS_FAST is kept as an alias). */
 enum { RTS_CROSS_EXACT = 0, RTS_CROSS_FAST = 1 };
 rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W, int H,
                        int D, int mode, void *stream);
 
 /* One full cross-based aggregation from a raw cost volume (no prefix volume
- * in HBM): local row prefix, segment sums and column sums in one kernel.
- * Results equal rts_hprefix + rts_cross_v(FAST) up to float rounding. */
+ * in HBM): each segment sum adds its costs in column order, each output adds
+ * its rows' segments in row order.  Equal to rts_hprefix + rts_cross_v in
+ * exact arithmetic (and bit-equal for integer costs); for float costs it
+ * avoids the rounding of the long row prefix, so it is closer to the exact
+ * value than the reference's dataflow. */
 rts_status rts_cross_fused(const float *cost, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W,
                            int H, int D, void *stream);
 

@@ -3,6 +3,8 @@
 // disparity-major [D][H][W].
 #include "rts_internal.h"
 
+#include <stdlib.h>
+
 #pragma clang fp contract(off)
 
 namespace rts {
@@ -22,10 +24,20 @@ __global__ void __launch_bounds__(256) k_box_f32(const float *__restrict__ in, f
     const size_t base = (size_t)d * plane(W, H);
     const int tw = 64 + 2 * r, th = BOX_TY + 2 * r;
     const int tid = threadIdx.y * 64 + threadIdx.x;
-    for (int i = tid; i < tw * th; i += 256) {
+    // All of a thread's loads are issued before its LDS stores.
+    constexpr int NB = ((BOX_TY + 2 * BOX_RMAX) * (64 + 2 * BOX_RMAX) + 255) / 256;
+    float v[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        const int i = tid + 256 * k;
         const int ty = i / tw, tx = i - ty * tw;
-        const int gy = wrapn(y0 - r + ty, H), gx = wrapn(x0 - r + tx, W);
-        tile[ty][tx] = in[base + (size_t)gy * W + gx];
+        v[k] = ty < th ? in[base + (size_t)wrapn(y0 - r + ty, H) * W + wrapn(x0 - r + tx, W)] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        const int i = tid + 256 * k;
+        const int ty = i / tw, tx = i - ty * tw;
+        if (ty < th) tile[ty][tx] = v[k];
     }
     __syncthreads();
     const float area = (float)((2 * r + 1) * (2 * r + 1));
@@ -186,17 +198,29 @@ __global__ void __launch_bounds__(64) k_hprefix(float *__restrict__ vol, long ro
     for (int c0 = 0; c0 < W; c0 += 64) {
         const int nc = min(64, W - c0);
         if (vec && nc == 64) {
+            // 16 independent float4 loads per lane, then the LDS stores.
             const int sub = lane >> 4, q = (lane & 15) * 4;
-            for (int i = sub; i < nr; i += 4) {
-                const float4 v = *reinterpret_cast<const float4 *>(vol + (r0 + i) * (long)W + c0 + q);
-                tile[i][q] = v.x;
-                tile[i][q + 1] = v.y;
-                tile[i][q + 2] = v.z;
-                tile[i][q + 3] = v.w;
+            float4 v[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int i = sub + 4 * k;
+                v[k] = i < nr ? *reinterpret_cast<const float4 *>(vol + (r0 + i) * (long)W + c0 + q)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int i = sub + 4 * k;
+                tile[i][q] = v[k].x;
+                tile[i][q + 1] = v[k].y;
+                tile[i][q + 2] = v[k].z;
+                tile[i][q + 3] = v[k].w;
             }
         } else {
-            for (int i = 0; i < nr; i++)
-                if (lane < nc) tile[i][lane] = vol[(r0 + i) * (long)W + c0 + lane];
+            float v[PF_ROWS];
+#pragma unroll
+            for (int i = 0; i < PF_ROWS; i++) v[i] = (i < nr && lane < nc) ? vol[(r0 + i) * (long)W + c0 + lane] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < PF_ROWS; i++) tile[i][lane] = v[i];
         }
         __syncthreads();
         if (lane < nr) {
@@ -232,192 +256,250 @@ __global__ void __launch_bounds__(64) k_hprefix(float *__restrict__ vol, long ro
 // ---------------------------------------------------------------- vertical sweep
 
 // seq/CrossBasedAggregation.cpp:127-169.  One block covers one d plane tile
-// of CV_TX columns x CV_TY rows.  Rows y0-LMAX .. y0+CV_TY+LMAX-1 of the
-// horizontal segment sums S(row, x) = P[row][x+r] - P[row][max(x-l-1, 0)] and
-// their lengths l+r+1 are built once in LDS (l, r = min of the ref arm at
-// (row, x) and the tar arm at (row, x-d)); each output then adds its rows.
-// EXACT adds S in the reference's row order in float; FAST prefix-sums S
-// down each column in FP64 and takes one difference.
+// of CV_TX columns x TY output rows and loads everything it needs up front
+// with independent, coalesced loads:
+//   - the input rows y0-17 .. y0+TY+16, columns x0-18 .. x0+CV_TX+16
+//     (clipped to the image) into LDS;
+//   - the ref arms at (row, x) and the tar arms at (row, x-d), combined into
+//     one packed word of minima per (row, column).
+// Segment sums S(row, x) over columns x-l .. x+r (with column 0 dropped when
+// x-l-1 < 0, seq/CrossBasedAggregation.cpp:151-156) then come from LDS, and
+// each output adds its rows' segments and lengths in the reference's row
+// order in float, then divides by the count.
+//   PREFIX input (the reference's dataflow): S = P[x+r] - P[max(x-l-1, 0)]
+//     of the row prefix P, bit-identical to seq/ cross_stereo_aggregation.
+//   DIRECT input (raw costs): S = sum of the costs in the segment, added in
+//     column order; no prefix volume, and each float sum has at most 35 terms.
+// Arms longer than the 17-row/column halo (possible only with L > 17) are
+// handled from global memory.
 constexpr int CV_TX = 64;
-constexpr int CV_TY = 32;
-constexpr int CV_LMAX = 17;   // tile halo: the reference's arm limit L (longer arms take a slow path)
+constexpr int CV_LMAX = 17;
+constexpr int CV_COLS = CV_TX + 2 * CV_LMAX + 1;   // x0-18 .. x0+80
 
-template <bool EXACT>
-__global__ void __launch_bounds__(256) k_cross_v(const float *__restrict__ P, float *__restrict__ out,
-                                                 const uint32_t *__restrict__ aref, const uint32_t *__restrict__ atar,
-                                                 int W, int H, int Lmax) {
-    constexpr int RMAX = CV_TY + 2 * CV_LMAX;
-    __shared__ float seg[EXACT ? RMAX : 1][CV_TX];
-    __shared__ double cum[EXACT ? 1 : RMAX + 1][CV_TX];
-    __shared__ int len[RMAX + 1][CV_TX];
-    const int x0 = blockIdx.x * CV_TX, y0 = blockIdx.y * CV_TY, d = blockIdx.z;
+template <bool DIRECT, int TY>
+__global__ void __launch_bounds__(256) k_cross_tile(const float *__restrict__ in, float *__restrict__ out,
+                                                    const uint32_t *__restrict__ aref,
+                                                    const uint32_t *__restrict__ atar, int W, int H) {
+    constexpr int R = TY + 2 * CV_LMAX;
+    constexpr int PER = (R + 3) / 4;                 // rows per thread in the segment pass
+    __shared__ float tile[R][CV_COLS + 1];           // input rows; later the segment sums
+    __shared__ uint32_t arm[R][CV_TX];               // packed minima: l, r, t, b
+    const int x0 = blockIdx.x * CV_TX, y0 = blockIdx.y * TY, d = blockIdx.z;
     const size_t base = (size_t)d * plane(W, H);
-    const int rlo = max(0, y0 - Lmax), rhi = min(H, y0 + CV_TY + Lmax);   // rows [rlo, rhi)
+    const int rlo = max(0, y0 - CV_LMAX), rhi = min(H, y0 + TY + CV_LMAX);
     const int nrows = rhi - rlo;
-    const int lx = threadIdx.x & 63, lg = threadIdx.x >> 6;   // column, row group (4)
+    const int clo = max(0, x0 - CV_LMAX - 1), chi = min(W, x0 + CV_TX + CV_LMAX);
+    const int ncols = chi - clo;
+    const int tid = threadIdx.x;
+    // Phase 1: bulk loads.  Every load of the thread is issued before any
+    // LDS store (the counts are compile-time), so they are all in flight
+    // together instead of one round trip per element.
+    constexpr int NT = (R * CV_COLS + 255) / 256, NA = (R * CV_TX + 255) / 256;
+    float tv[NT];
+    uint32_t av[NA], bv[NA];
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        const int e = tid + 256 * k;
+        const int rr = e / CV_COLS, cc = e - rr * CV_COLS;
+        tv[k] = (rr < nrows && cc < ncols) ? in[base + (size_t)(rlo + rr) * W + clo + cc] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        const int e = tid + 256 * k;
+        const int rr = min(e >> 6, nrows - 1), cc = e & 63;
+        const int x = min(x0 + cc, W - 1);
+        const size_t ro = (size_t)(rlo + rr) * W;
+        av[k] = aref[ro + x];
+        bv[k] = atar[ro + wrap1(x - d, W)];
+    }
+#pragma unroll
+    for (int k = 0; k < NT; k++) {
+        const int e = tid + 256 * k;
+        const int rr = e / CV_COLS, cc = e - rr * CV_COLS;
+        if (rr < R) tile[rr][cc] = tv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NA; k++) {
+        const int e = tid + 256 * k;
+        const int rr = e >> 6, cc = e & 63;
+        const uint32_t a = av[k], b = bv[k];
+        if (rr < R)
+            arm[rr][cc] = (uint32_t)min(arm_l(a), arm_l(b)) | ((uint32_t)min(arm_r(a), arm_r(b)) << 8) |
+                          ((uint32_t)min(arm_t(a), arm_t(b)) << 16) | ((uint32_t)min(arm_b(a), arm_b(b)) << 24);
+    }
+    __syncthreads();
+    // Phase 2: segment sums of (row, column) into registers.
+    const int lx = tid & 63, lg = tid >> 6;
     const int x = x0 + lx;
     const bool colok = x < W;
     const int tx = colok ? wrap1(x - d, W) : 0;
-    // Segment sums per (row, column).
-    for (int i = lg; i < nrows; i += 4) {
-        float s = 0.0f;
-        int n = 0;
-        if (colok) {
-            const int row = rlo + i;
-            const uint32_t ar = aref[(size_t)row * W + x], at = atar[(size_t)row * W + tx];
-            const int l = min(arm_l(ar), arm_l(at)), r = min(arm_r(ar), arm_r(at));
-            // x-l-1 < 0 clamps to column 0 (the reference then drops column
-            // 0's cost, seq/CrossBasedAggregation.cpp:151-156); x+r never
-            // leaves the image for arms from findCross.
+    float sv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = lg + 4 * k;
+        float sum = 0.0f;
+        if (colok && i < nrows) {
+            const uint32_t a = arm[i][lx];
+            const int l = arm_l(a), r = arm_r(a);
             const int xl = max(x - l - 1, 0), xr = min(x + r, W - 1);
-            const float *Pr = P + base + (size_t)row * W;
-            s = Pr[xr] - Pr[xl];
-            n = l + r + 1;
-        }
-        if (EXACT) {
-            seg[i][lx] = s;
-            len[i][lx] = n;
-        } else {
-            cum[i + 1][lx] = (double)s;
-            len[i + 1][lx] = n;
-        }
-    }
-    __syncthreads();
-    if (!EXACT) {
-        // Column prefix sums (FP64 segments, int lengths); one wave per 16 columns.
-        if (threadIdx.x < CV_TX) {
-            double a = 0.0;
-            int c = 0;
-            cum[0][lx] = 0.0;
-            len[0][lx] = 0;
-            for (int i = 1; i <= nrows; i++) {
-                a += cum[i][lx];
-                c += len[i][lx];
-                cum[i][lx] = a;
-                len[i][lx] = c;
+            if (xl >= clo && xr < chi) {
+                if (DIRECT) {
+                    for (int c = xl + 1 - clo; c <= xr - clo; c++) sum += tile[i][c];
+                } else {
+                    sum = tile[i][xr - clo] - tile[i][xl - clo];
+                }
+            } else {
+                const float *rowp = in + base + (size_t)(rlo + i) * W;
+                if (DIRECT) {
+                    for (int c = xl + 1; c <= xr; c++) sum += rowp[c];
+                } else {
+                    sum = rowp[xr] - rowp[xl];
+                }
             }
         }
-        __syncthreads();
+        sv[k] = sum;
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = lg + 4 * k;
+        if (i < nrows) tile[i][lx] = sv[k];
+    }
+    __syncthreads();
     if (!colok) return;
-    for (int ly = lg; ly < CV_TY; ly += 4) {
+    // Phase 3: each output adds its rows in order.
+    for (int ly = lg; ly < TY; ly += 4) {
         const int y = y0 + ly;
         if (y >= H) break;
-        const uint32_t ar = aref[(size_t)y * W + x], at = atar[(size_t)y * W + tx];
-        const int t = min(arm_t(ar), arm_t(at)), b = min(arm_b(ar), arm_b(at));
-        const int ia = y - t - rlo, ib = y + b - rlo;   // inclusive row range in the tile
-        float v;
-        if (ia < 0 || ib >= nrows) {
-            // Arms longer than the tile halo (never produced by rts_find_cross
-            // with L <= CV_LMAX): sum the rows straight from global memory,
-            // skipping rows outside the image as the reference does.
-            float sum = 0.0f;
-            int cnt = 0;
+        const uint32_t a = arm[y - rlo][lx];
+        const int t = arm_t(a), b = arm_b(a);
+        const int ia = y - t - rlo, ib = y + b - rlo;
+        float sum = 0.0f;
+        int cnt = 0;
+        if (ia >= 0 && ib < nrows) {
+            for (int i = ia; i <= ib; i++) {
+                const uint32_t ai = arm[i][lx];
+                sum += tile[i][lx];
+                cnt += arm_l(ai) + arm_r(ai) + 1;
+            }
+        } else {
+            // Vertical arms beyond the halo: rows from global memory.
             for (int row = max(y - t, 0); row <= min(y + b, H - 1); row++) {
                 const uint32_t rr = aref[(size_t)row * W + x], rt = atar[(size_t)row * W + tx];
                 const int l = min(arm_l(rr), arm_l(rt)), r = min(arm_r(rr), arm_r(rt));
                 const int xl = max(x - l - 1, 0), xr = min(x + r, W - 1);
-                const float *Pr = P + base + (size_t)row * W;
-                sum += Pr[xr] - Pr[xl];
+                const float *rowp = in + base + (size_t)row * W;
+                float seg = 0.0f;
+                if (DIRECT) {
+                    for (int c = xl + 1; c <= xr; c++) seg += rowp[c];
+                } else {
+                    seg = rowp[xr] - rowp[xl];
+                }
+                sum += seg;
                 cnt += l + r + 1;
             }
-            out[base + (size_t)y * W + x] = __fdiv_rn(sum, (float)cnt);
-            continue;
         }
-        if (EXACT) {
-            float sum = 0.0f;
-            int cnt = 0;
-            for (int i = ia; i <= ib; i++) {
-                sum += seg[i][lx];
-                cnt += len[i][lx];
-            }
-            v = __fdiv_rn(sum, (float)cnt);
-        } else {
-            const double sum = cum[ib + 1][lx] - cum[ia][lx];
-            const int cnt = len[ib + 1][lx] - len[ia][lx];
-            v = (float)(sum / (double)cnt);
-        }
-        out[base + (size_t)y * W + x] = v;
+        out[base + (size_t)y * W + x] = __fdiv_rn(sum, (float)cnt);
     }
 }
 
-// One whole cross-based aggregation from a raw cost volume.  The block's
-// cost tile (rows y0-L .. y0+CV_TY+L-1, columns x0-L-1 .. x0+CV_TX+L-1,
-// clipped to the image) is prefix-summed along each row in LDS, starting at
-// the tile's first column; segment sums are differences of that local
-// prefix (the same sums as the reference's global prefix, including the
-// column-0 drop when the tile starts at column 0), then prefix-summed down
-// each column in FP64 as the FAST sweep does.  Integer-valued costs stay
-// exact throughout.
-constexpr int CF_COLS = CV_TX + 2 * CV_LMAX + 1;
-constexpr int CF_ROWS = CV_TY + 2 * CV_LMAX;
+// The prefix-input sweep (the reference's dataflow) without an LDS copy of
+// the input: each wave owns a quarter of the tile's rows; for a row its 64
+// lanes load the 99 prefix values x0-18 .. x0+80 into two registers (lane j
+// holds columns x0-18+j and x0+46+j) together with the ref and tar arms of
+// their column, and a segment's two endpoints P[x+r] and P[max(x-l-1, 0)]
+// are fetched from the owning lanes with ds_bpermute.  Rows are loaded in
+// batches of PV_BATCH so their loads overlap.  Segment sums and the combined
+// arm minima go to LDS; each output then adds its rows' segments in the
+// reference's row order and divides by the count, bit-identical to
+// seq/CrossBasedAggregation.cpp:127-169.
+constexpr int PV_TY = 64;
+constexpr int PV_R = PV_TY + 2 * CV_LMAX;
+constexpr int PV_PER = (PV_R + 3) / 4;
+constexpr int PV_BATCH = 5;
 
-__global__ void __launch_bounds__(256) k_cross_fused(const float *__restrict__ C, float *__restrict__ out,
-                                                     const uint32_t *__restrict__ aref,
-                                                     const uint32_t *__restrict__ atar, int W, int H) {
-    __shared__ float tile[CF_ROWS][CF_COLS + 1];
-    __shared__ double cum[CF_ROWS + 1][CV_TX];
-    __shared__ short len[CF_ROWS + 1][CV_TX];
-    const int x0 = blockIdx.x * CV_TX, y0 = blockIdx.y * CV_TY, d = blockIdx.z;
+__device__ __forceinline__ float lane_fetch(float v0, float v1, int j) {
+    // value of column j (0..98) of the row segment held as (v0 lanes 0..63, v1 lanes 0..34)
+    const int src = (j & 63) << 2;
+    const float a = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(v0)));
+    const float b = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(v1)));
+    return j < 64 ? a : b;
+}
+
+__global__ void __launch_bounds__(256) k_cross_pv(const float *__restrict__ P, float *__restrict__ out,
+                                                  const uint32_t *__restrict__ aref, const uint32_t *__restrict__ atar,
+                                                  int W, int H) {
+    __shared__ float segS[PV_R][CV_TX];
+    __shared__ uint32_t armc[PV_R][CV_TX];
+    const int x0 = blockIdx.x * CV_TX, y0 = blockIdx.y * PV_TY, d = blockIdx.z;
     const size_t base = (size_t)d * plane(W, H);
-    const int rlo = max(0, y0 - CV_LMAX), rhi = min(H, y0 + CV_TY + CV_LMAX);
-    const int clo = max(0, x0 - CV_LMAX - 1), chi = min(W, x0 + CV_TX + CV_LMAX);
-    const int nrows = rhi - rlo, ncols = chi - clo;
-    for (int i = threadIdx.x; i < nrows * ncols; i += 256) {
-        const int rr = i / ncols, cc = i - rr * ncols;
-        tile[rr][cc] = C[base + (size_t)(rlo + rr) * W + clo + cc];
-    }
-    __syncthreads();
-    for (int rr = threadIdx.x; rr < nrows; rr += 256) {
-        float a = tile[rr][0];
-        for (int cc = 1; cc < ncols; cc++) {
-            a = a + tile[rr][cc];
-            tile[rr][cc] = a;
-        }
-    }
-    __syncthreads();
-    const int lx = threadIdx.x & 63, lg = threadIdx.x >> 6;
-    const int x = x0 + lx;
+    const int rlo = max(0, y0 - CV_LMAX), rhi = min(H, y0 + PV_TY + CV_LMAX);
+    const int nrows = rhi - rlo;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = x0 + lane;
     const bool colok = x < W;
-    const int tx = colok ? wrap1(x - d, W) : 0;
-    for (int i = lg; i < nrows; i += 4) {
-        double s = 0.0;
-        int n = 0;
-        if (colok) {
-            const int row = rlo + i;
-            const uint32_t ar = aref[(size_t)row * W + x], at = atar[(size_t)row * W + tx];
-            const int l = min(min(arm_l(ar), arm_l(at)), CV_LMAX), r = min(min(arm_r(ar), arm_r(at)), CV_LMAX);
-            const int xl = max(x - l - 1, 0) - clo, xr = min(x + r, W - 1) - clo;
-            s = (double)(tile[i][xr] - tile[i][xl]);
-            n = l + r + 1;
+    const int xc = min(x, W - 1);
+    const int tx = wrap1(xc - d, W);
+    const int cs = x0 - CV_LMAX - 1;   // first column of the row segment
+    const int c0 = cs + lane, c1 = cs + 64 + lane;
+    const bool ok0 = c0 >= 0 && c0 < W, ok1 = lane < CV_COLS - 64 && c1 >= 0 && c1 < W;
+#pragma unroll
+    for (int b0 = 0; b0 < PV_PER; b0 += PV_BATCH) {
+        float p0[PV_BATCH], p1[PV_BATCH];
+        uint32_t ar[PV_BATCH], at[PV_BATCH];
+#pragma unroll
+        for (int k = 0; k < PV_BATCH; k++) {
+            const int i = w + 4 * (b0 + k);
+            const int row = rlo + min(i, nrows - 1);
+            const size_t ro = (size_t)row * W;
+            p0[k] = ok0 ? P[base + ro + c0] : 0.0f;
+            p1[k] = ok1 ? P[base + ro + c1] : 0.0f;
+            ar[k] = aref[ro + xc];
+            at[k] = atar[ro + tx];
         }
-        cum[i + 1][lx] = s;
-        len[i + 1][lx] = (short)n;
-    }
-    __syncthreads();
-    if (threadIdx.x < CV_TX) {
-        double a = 0.0;
-        int c = 0;
-        cum[0][lx] = 0.0;
-        len[0][lx] = 0;
-        for (int i = 1; i <= nrows; i++) {
-            a += cum[i][lx];
-            c += len[i][lx];
-            cum[i][lx] = a;
-            len[i][lx] = (short)c;
+#pragma unroll
+        for (int k = 0; k < PV_BATCH; k++) {
+            const int i = w + 4 * (b0 + k);
+            const uint32_t a = ar[k], t = at[k];
+            const int l = min(arm_l(a), arm_l(t)), r = min(arm_r(a), arm_r(t));
+            // x-l-1 < 0 clamps to column 0 (the reference then drops column
+            // 0's cost, seq/CrossBasedAggregation.cpp:151-156).
+            const int xl = max(xc - l - 1, 0), xr = min(xc + r, W - 1);
+            const float vr = lane_fetch(p0[k], p1[k], xr - cs);
+            const float vl = lane_fetch(p0[k], p1[k], xl - cs);
+            if (i < nrows) {
+                segS[i][lane] = vr - vl;
+                armc[i][lane] = (uint32_t)l | ((uint32_t)r << 8) | ((uint32_t)min(arm_t(a), arm_t(t)) << 16) |
+                                ((uint32_t)min(arm_b(a), arm_b(t)) << 24);
+            }
         }
     }
     __syncthreads();
     if (!colok) return;
-    for (int ly = lg; ly < CV_TY; ly += 4) {
+    for (int ly = w; ly < PV_TY; ly += 4) {
         const int y = y0 + ly;
         if (y >= H) break;
-        const uint32_t ar = aref[(size_t)y * W + x], at = atar[(size_t)y * W + tx];
-        const int t = min(arm_t(ar), arm_t(at)), b = min(arm_b(ar), arm_b(at));
-        const int ia = max(y - t, rlo) - rlo, ib = min(y + b, rhi - 1) - rlo;
-        const double sum = cum[ib + 1][lx] - cum[ia][lx];
-        const int cnt = len[ib + 1][lx] - len[ia][lx];
-        out[base + (size_t)y * W + x] = (float)(sum / (double)cnt);
+        const uint32_t a = armc[y - rlo][lane];
+        const int ia = y - arm_t(a) - rlo, ib = y + arm_b(a) - rlo;
+        float sum = 0.0f;
+        int cnt = 0;
+        if (ia >= 0 && ib < nrows) {
+            for (int i = ia; i <= ib; i++) {
+                const uint32_t ai = armc[i][lane];
+                sum += segS[i][lane];
+                cnt += arm_l(ai) + arm_r(ai) + 1;
+            }
+        } else {
+            // Vertical arms beyond the halo (only with L > 17): global memory.
+            for (int row = max(ia + rlo, 0); row <= min(ib + rlo, H - 1); row++) {
+                const uint32_t rr = aref[(size_t)row * W + x], rt = atar[(size_t)row * W + tx];
+                const int l = min(arm_l(rr), arm_l(rt)), r = min(arm_r(rr), arm_r(rt));
+                const float *Pr = P + base + (size_t)row * W;
+                sum += Pr[min(x + r, W - 1)] - Pr[max(x - l - 1, 0)];
+                cnt += l + r + 1;
+            }
+        }
+        out[base + (size_t)y * W + x] = __fdiv_rn(sum, (float)cnt);
     }
 }
 
@@ -425,17 +507,30 @@ __global__ void __launch_bounds__(256) k_cross_fused(const float *__restrict__ C
 
 using namespace rts;
 
+template <bool DIRECT>
+static int launch_cross_tile(const float *in, float *out, const uint32_t *ar, const uint32_t *at, int W, int H, int D,
+                             hipStream_t s) {
+    static const int ty = [] {
+        const char *e = getenv("RTS_CROSS_TY");
+        return (e && atoi(e) == 64) ? 64 : 32;
+    }();
+    RTS_CHECK(div_up(H, ty) <= 65535, "height %d too large", H);
+    dim3 grid(div_up(W, CV_TX), div_up(H, ty), D);
+    if (ty == 64)
+        k_cross_tile<DIRECT, 64><<<grid, 256, 0, s>>>(in, out, ar, at, W, H);
+    else
+        k_cross_tile<DIRECT, 32><<<grid, 256, 0, s>>>(in, out, ar, at, W, H);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
 extern "C" {
 
 rts_status rts_cross_fused(const float *cost, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W,
                            int H, int D, void *stream) {
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(cost != out, "cross aggregation is out of place");
-    RTS_CHECK(div_up(H, CV_TY) <= 65535, "height %d too large", H);
-    dim3 grid(div_up(W, CV_TX), div_up(H, CV_TY), D);
-    k_cross_fused<<<grid, 256, 0, S(stream)>>>(cost, out, arms_ref, arms_tar, W, H);
-    RTS_LAUNCHED();
-    return RTS_OK;
+    return launch_cross_tile<true>(cost, out, arms_ref, arms_tar, W, H, D, S(stream));
 }
 
 rts_status rts_box(const float *in, float *out, int W, int H, int D, int r, void *stream) {
@@ -517,17 +612,16 @@ rts_status rts_hprefix(float *vol, int W, int H, int D, void *stream) {
 
 // Largest arm in either map bounds the tile halo; arms above CV_LMAX are
 // rejected (the reference's arms are at most 17).
+
 rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W, int H,
                        int D, int mode, void *stream) {
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(P != out, "cross aggregation is out of place");
     RTS_CHECK(mode == RTS_CROSS_EXACT || mode == RTS_CROSS_FAST, "bad cross mode %d", mode);
-    RTS_CHECK(div_up(H, CV_TY) <= 65535, "height %d too large", H);
-    dim3 grid(div_up(W, CV_TX), div_up(H, CV_TY), D);
-    if (mode == RTS_CROSS_EXACT)
-        k_cross_v<true><<<grid, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, CV_LMAX);
-    else
-        k_cross_v<false><<<grid, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, CV_LMAX);
+    // Both mode values run the same sweep, which adds in the reference's order.
+    RTS_CHECK(div_up(H, PV_TY) <= 65535, "height %d too large", H);
+    dim3 grid(div_up(W, CV_TX), div_up(H, PV_TY), D);
+    k_cross_pv<<<grid, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H);
     RTS_LAUNCHED();
     return RTS_OK;
 }

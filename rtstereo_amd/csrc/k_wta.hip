@@ -5,61 +5,72 @@
 
 namespace rts {
 
-// One block per image row.  For every d the block stages the row vol[d][y][:]
-// in LDS (double-buffered), then each thread updates the running best of its
-// pixels both for the left view (value at x) and for the right view (value
-// at (x+d) mod W, i.e. the right pixel x matched with left pixel x+d).  Strict
-// comparisons from d=0 up keep the lowest d on ties (seq/WTA.cpp:16-21).
-// After the sweep the row's D_L and D_R sit in LDS and the left-right check
-// is evaluated there (seq/CrossCheck.cpp:15-21).
+// One block per image row; thread t owns pixels t, t+256, ...  For every d
+// it reads vol[d][y][x] for the left view and vol[d][y][(x+d) mod W] for the
+// right view (right pixel x matched with left pixel x+d); both reads are
+// contiguous across the wavefront, and the second mostly hits the cache
+// lines the first brought in.  Strict comparisons from d=0 up keep the
+// lowest d on ties (seq/WTA.cpp:16-21).  WTA_DU disparities are loaded
+// before they are compared, so their loads are in flight together.  After
+// the sweep the row's D_L and D_R go to LDS and the left-right check
+// (seq/CrossCheck.cpp:15-21) runs there.
 constexpr int WTA_THREADS = 256;
+constexpr int WTA_DU = 4;
 
 template <int PPT, bool MAXIMIZE>
 __global__ void __launch_bounds__(WTA_THREADS) k_wta(const float *__restrict__ vol, uint8_t *__restrict__ dl,
                                                      uint8_t *__restrict__ dr, uint8_t *__restrict__ mask,
                                                      uint8_t *__restrict__ beat0, int W, int H, int D, int tol) {
-    extern __shared__ float sh[];   // 2*W floats of rows, then 2*W bytes of maps
-    float *rowbuf = sh;
-    uint8_t *mapL = reinterpret_cast<uint8_t *>(sh + 2 * W);
-    uint8_t *mapR = mapL + W;
+    extern __shared__ uint8_t maps[];   // 2*W bytes: D_L then D_R of the row
+    uint8_t *mapL = maps, *mapR = maps + W;
     const int y = blockIdx.x;
     const size_t pl = plane(W, H), rowoff = (size_t)y * W;
     const bool want_r = dr != nullptr;
+    const float *row = vol + rowoff;
     float bestL[PPT], bestR[PPT];
-    int dL[PPT], dR[PPT];
-    for (int d = 0; d < D; d++) {
-        float *buf = rowbuf + (d & 1) * W;
-        const float *src = vol + (size_t)d * pl + rowoff;
-        for (int x = threadIdx.x; x < W; x += WTA_THREADS) buf[x] = src[x];
-        __syncthreads();
+    int dL[PPT], dR[PPT], xs[PPT];
 #pragma unroll
-        for (int i = 0; i < PPT; i++) {
-            const int x = threadIdx.x + i * WTA_THREADS;
-            if (x < W) {
-                const float v = buf[x];
-                if (d == 0) {
-                    bestL[i] = v;
-                    dL[i] = 0;
-                } else if (MAXIMIZE ? (v > bestL[i]) : (v < bestL[i])) {
+    for (int i = 0; i < PPT; i++) {
+        xs[i] = min((int)threadIdx.x + i * WTA_THREADS, W - 1);
+        bestL[i] = bestR[i] = 0.0f;
+        dL[i] = dR[i] = 0;
+    }
+    for (int d0 = 0; d0 < D; d0 += WTA_DU) {
+        float vL[WTA_DU][PPT], vR[WTA_DU][PPT];
+#pragma unroll
+        for (int j = 0; j < WTA_DU; j++) {
+            const int d = min(d0 + j, D - 1);
+            const float *pr = row + (size_t)d * pl;
+#pragma unroll
+            for (int i = 0; i < PPT; i++) {
+                vL[j][i] = pr[xs[i]];
+                if (want_r) {
+                    int xr = xs[i] + d;
+                    if (xr >= W) xr -= W;
+                    vR[j][i] = pr[xr];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < WTA_DU; j++) {
+            const int d = d0 + j;
+            if (d >= D) break;
+#pragma unroll
+            for (int i = 0; i < PPT; i++) {
+                const float v = vL[j][i];
+                if (d == 0 || (MAXIMIZE ? (v > bestL[i]) : (v < bestL[i]))) {
                     bestL[i] = v;
                     dL[i] = d;
                 }
                 if (want_r) {
-                    int xr = x + d;
-                    if (xr >= W) xr -= W;
-                    const float u = buf[xr];
-                    if (d == 0) {
-                        bestR[i] = u;
-                        dR[i] = 0;
-                    } else if (MAXIMIZE ? (u > bestR[i]) : (u < bestR[i])) {
+                    const float u = vR[j][i];
+                    if (d == 0 || (MAXIMIZE ? (u > bestR[i]) : (u < bestR[i]))) {
                         bestR[i] = u;
                         dR[i] = d;
                     }
                 }
             }
         }
-        // The next iteration writes the other buffer; the one after that
-        // rewrites this one only after the next barrier.
     }
 #pragma unroll
     for (int i = 0; i < PPT; i++) {
@@ -209,8 +220,7 @@ template <bool MAXIMIZE>
 static int launch_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, uint8_t *beat, int W, int H, int D,
                       int tol, hipStream_t s) {
     const int ppt = div_up(W, WTA_THREADS);
-    const size_t shm = (size_t)2 * W * sizeof(float) + 2 * (size_t)W;
-    RTS_CHECK(shm <= 64 * 1024, "row of %d pixels too wide for the WTA kernel", W);
+    const size_t shm = 2 * (size_t)W;
 #define RTS_WTA_CASE(N)                                                                         \
     case N:                                                                                     \
         k_wta<N, MAXIMIZE><<<H, WTA_THREADS, shm, s>>>(vol, dl, dr, mask, beat, W, H, D, tol); \
