@@ -13,8 +13,6 @@ Prints one JSON line.  Options:
   --gpus N      only 1 is supported (one GPU per box)
   --steps K     timed frames (default 30)
   --warmup W    untimed frames (default 3)
-  --mode        exact (bit-exact with the reference's float order) or fast
-                (FP64 column sums, <= 1e-5 relative); default fast
   --cpu-seconds target length of the CPU baseline sample (default 12; 0 = skip)
 """
 from __future__ import annotations
@@ -91,7 +89,6 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-copy-probe", action="store_true")
     args = ap.parse_args()
@@ -110,7 +107,7 @@ def main() -> int:
     pair = gen.census_cross_1080p()
     stream = Stream()
     left, right = DeviceArray.from_host(pair.left), DeviceArray.from_host(pair.right)
-    desc = PL.preset("census_cross", W, H, D, cross_exact=int(args.mode == "exact"))
+    desc = PL.preset("census_cross", W, H, D)
     pipe = PL.Pipeline(desc)
     pipe.enable_timing(True)
 
@@ -180,7 +177,7 @@ def main() -> int:
         "config": {"workload": "census9x7+colour AD combined (lambda 30/10) -> cross-based aggregation "
                                "(L 17, tao 20, one H/V pass) -> WTA + right view + L/R check (tol 1), "
                                f"{W}x{H}, D={D}",
-                   "mode": args.mode, "width": W, "height": H, "disparities": D, "pd_per_frame": pd},
+                   "arithmetic": "bit-identical to the reference's float order", "width": W, "height": H, "disparities": D, "pd_per_frame": pd},
         "frames_per_s": 1e3 / (total_ms / args.steps),
         "latency_ms": {"p50": float(np.percentile(frame_ms, 50)), "p99": float(np.percentile(frame_ms, 99)),
                        "max": float(frame_ms.max())},

@@ -403,103 +403,135 @@ __global__ void __launch_bounds__(256) k_cross_tile(const float *__restrict__ in
     }
 }
 
-// The prefix-input sweep (the reference's dataflow) without an LDS copy of
-// the input: each wave owns a quarter of the tile's rows; for a row its 64
-// lanes load the 99 prefix values x0-18 .. x0+80 into two registers (lane j
-// holds columns x0-18+j and x0+46+j) together with the ref and tar arms of
-// their column, and a segment's two endpoints P[x+r] and P[max(x-l-1, 0)]
-// are fetched from the owning lanes with ds_bpermute.  Rows are loaded in
-// batches of PV_BATCH so their loads overlap.  Segment sums and the combined
-// arm minima go to LDS; each output then adds its rows' segments in the
-// reference's row order and divides by the count, bit-identical to
-// seq/CrossBasedAggregation.cpp:127-169.
-constexpr int PV_TY = 64;
-constexpr int PV_R = PV_TY + 2 * CV_LMAX;
-constexpr int PV_PER = (PV_R + 3) / 4;
-constexpr int PV_BATCH = 5;
+// The prefix-input sweep streamed down a column strip.  A block owns 64
+// columns and a segment of SEG output rows of one d plane and walks the rows
+// of [seg-17, seg+SEG+17) in steps of 16: each step's 16 rows are loaded (the
+// 99 prefix values a row's segments can touch, held as two registers per
+// lane, plus the ref and tar arms of the lane's column), their segment sums
+// S and packed (length, top, bottom) words go into a 64-row LDS ring, and
+// every output row whose vertical support is complete is emitted: its
+// segments added in the reference's row order, then divided by the count
+// (seq/CrossBasedAggregation.cpp:127-169, bit-identical).  Each segment sum
+// is computed once per row (no tile halo), and the next step's loads are in
+// flight while the current outputs are summed.
+constexpr int SW_CH = 16;        // rows per step (4 per wave)
+constexpr int SW_RING = 64;      // ring rows (power of two, >= SW_CH + 2*17 + SW_CH)
+constexpr int SW_SEG = 136;      // output rows per block
 
+// Column j (0..98) of a row segment held by a wave as two registers per
+// lane (lanes 0..63: columns 0..63, lanes 0..34: columns 64..98), fetched
+// from the owning lane with ds_bpermute.
 __device__ __forceinline__ float lane_fetch(float v0, float v1, int j) {
-    // value of column j (0..98) of the row segment held as (v0 lanes 0..63, v1 lanes 0..34)
     const int src = (j & 63) << 2;
     const float a = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(v0)));
     const float b = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(v1)));
     return j < 64 ? a : b;
 }
 
-__global__ void __launch_bounds__(256) k_cross_pv(const float *__restrict__ P, float *__restrict__ out,
-                                                  const uint32_t *__restrict__ aref, const uint32_t *__restrict__ atar,
-                                                  int W, int H) {
-    __shared__ float segS[PV_R][CV_TX];
-    __shared__ uint32_t armc[PV_R][CV_TX];
-    const int x0 = blockIdx.x * CV_TX, y0 = blockIdx.y * PV_TY, d = blockIdx.z;
+__device__ __forceinline__ uint32_t min_u8x4(uint32_t a, uint32_t b) {
+    // per-byte minimum of two packed arm words via two 16-bit pair minima
+    typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+    const uint32_t alo = a & 0x00FF00FFu, blo = b & 0x00FF00FFu;
+    const uint32_t ahi = (a >> 8) & 0x00FF00FFu, bhi = (b >> 8) & 0x00FF00FFu;
+    us2 x0 = __builtin_bit_cast(us2, alo), y0 = __builtin_bit_cast(us2, blo);
+    us2 x1 = __builtin_bit_cast(us2, ahi), y1 = __builtin_bit_cast(us2, bhi);
+    const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(x0, y0));
+    const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(x1, y1));
+    return lo | (hi << 8);
+}
+
+__global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P, float *__restrict__ out,
+                                                     const uint32_t *__restrict__ aref,
+                                                     const uint32_t *__restrict__ atar, int W, int H, int nx,
+                                                     int nseg, int D) {
+    __shared__ float2 ring[SW_RING][CV_TX];   // x: segment sum, y: bits of (len | top<<10 | bottom<<18)
+    // Work order (segment, d, x-tile) with x fastest, cut into one
+    // contiguous range per XCD: x-neighbours (which share 35 prefix columns)
+    // run on the same L2.
+    const int wk = xcd_order(blockIdx.x, nx * nseg * D);
+    const int xt = wk % nx, rest = wk / nx;
+    const int d = rest % D, sg = rest / D;
+    const int x0 = xt * CV_TX, s0 = sg * SW_SEG;
+    const int s1 = min(H, s0 + SW_SEG);
     const size_t base = (size_t)d * plane(W, H);
-    const int rlo = max(0, y0 - CV_LMAX), rhi = min(H, y0 + PV_TY + CV_LMAX);
-    const int nrows = rhi - rlo;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int rlo = max(0, s0 - CV_LMAX), rhi = min(H, s1 + CV_LMAX);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform: scalar row bases
     const int x = x0 + lane;
     const bool colok = x < W;
     const int xc = min(x, W - 1);
     const int tx = wrap1(xc - d, W);
-    const int cs = x0 - CV_LMAX - 1;   // first column of the row segment
+    const int cs = x0 - CV_LMAX - 1;
     const int c0 = cs + lane, c1 = cs + 64 + lane;
     const bool ok0 = c0 >= 0 && c0 < W, ok1 = lane < CV_COLS - 64 && c1 >= 0 && c1 < W;
+    const float *Pb = P + base;
+    float p0[4], p1[4];
+    uint32_t ar[4], at[4];
+    auto load_rows = [&](int r) {
 #pragma unroll
-    for (int b0 = 0; b0 < PV_PER; b0 += PV_BATCH) {
-        float p0[PV_BATCH], p1[PV_BATCH];
-        uint32_t ar[PV_BATCH], at[PV_BATCH];
-#pragma unroll
-        for (int k = 0; k < PV_BATCH; k++) {
-            const int i = w + 4 * (b0 + k);
-            const int row = rlo + min(i, nrows - 1);
+        for (int k = 0; k < 4; k++) {
+            const int row = min(r + w + 4 * k, rhi - 1);
             const size_t ro = (size_t)row * W;
-            p0[k] = ok0 ? P[base + ro + c0] : 0.0f;
-            p1[k] = ok1 ? P[base + ro + c1] : 0.0f;
+            p0[k] = ok0 ? Pb[ro + c0] : 0.0f;
+            p1[k] = ok1 ? Pb[ro + c1] : 0.0f;
             ar[k] = aref[ro + xc];
             at[k] = atar[ro + tx];
         }
+    };
+    int out_next = s0;   // first output row not yet written
+    load_rows(rlo);
+    for (int r = rlo; r < rhi; r += SW_CH) {
+        // segment sums of rows r .. r+15 into the ring
 #pragma unroll
-        for (int k = 0; k < PV_BATCH; k++) {
-            const int i = w + 4 * (b0 + k);
-            const uint32_t a = ar[k], t = at[k];
-            const int l = min(arm_l(a), arm_l(t)), r = min(arm_r(a), arm_r(t));
-            // x-l-1 < 0 clamps to column 0 (the reference then drops column
-            // 0's cost, seq/CrossBasedAggregation.cpp:151-156).
-            const int xl = max(xc - l - 1, 0), xr = min(xc + r, W - 1);
-            const float vr = lane_fetch(p0[k], p1[k], xr - cs);
-            const float vl = lane_fetch(p0[k], p1[k], xl - cs);
-            if (i < nrows) {
-                segS[i][lane] = vr - vl;
-                armc[i][lane] = (uint32_t)l | ((uint32_t)r << 8) | ((uint32_t)min(arm_t(a), arm_t(t)) << 16) |
-                                ((uint32_t)min(arm_b(a), arm_b(t)) << 24);
+        for (int k = 0; k < 4; k++) {
+            const int row = r + w + 4 * k;
+            const uint32_t a = min_u8x4(ar[k], at[k]);
+            const int l = arm_l(a), rr = arm_r(a);
+            float sv;
+            if (l <= CV_LMAX && rr <= CV_LMAX) {
+                const int xl = max(xc - l - 1, 0), xr = min(xc + rr, W - 1);
+                sv = lane_fetch(p0[k], p1[k], xr - cs) - lane_fetch(p0[k], p1[k], xl - cs);
+            } else {
+                // horizontal arm longer than the segment window (L > 17)
+                const float *Pr = Pb + (size_t)min(row, rhi - 1) * W;
+                sv = Pr[min(xc + rr, W - 1)] - Pr[max(xc - l - 1, 0)];
             }
+            if (row < rhi)
+                ring[row & (SW_RING - 1)][lane] = make_float2(
+                    sv, __uint_as_float((uint32_t)(l + rr + 1) | ((a >> 16) & 0xFFu) << 10 | (a >> 24) << 18));
         }
-    }
-    __syncthreads();
-    if (!colok) return;
-    for (int ly = w; ly < PV_TY; ly += 4) {
-        const int y = y0 + ly;
-        if (y >= H) break;
-        const uint32_t a = armc[y - rlo][lane];
-        const int ia = y - arm_t(a) - rlo, ib = y + arm_b(a) - rlo;
-        float sum = 0.0f;
-        int cnt = 0;
-        if (ia >= 0 && ib < nrows) {
-            for (int i = ia; i <= ib; i++) {
-                const uint32_t ai = armc[i][lane];
-                sum += segS[i][lane];
-                cnt += arm_l(ai) + arm_r(ai) + 1;
+        __syncthreads();
+        const int rnext = r + SW_CH;
+        if (rnext < rhi) load_rows(rnext);
+        // outputs whose rows are all in the ring
+        const int avail = min(rnext, rhi);
+        const int out_end = avail >= rhi ? s1 : min(s1, avail - CV_LMAX);
+        if (colok)
+            for (int y = out_next + w; y < out_end; y += 4) {
+                const uint32_t word = __float_as_uint(ring[y & (SW_RING - 1)][lane].y);
+                const int t = (word >> 10) & 0xFF, b = word >> 18;
+                float sum = 0.0f;
+                int cnt = 0;
+                if (t <= CV_LMAX && b <= CV_LMAX) {
+                    for (int i = y - t; i <= y + b; i++) {
+                        const float2 e = ring[i & (SW_RING - 1)][lane];
+                        sum += e.x;
+                        cnt += __float_as_uint(e.y) & 0x3FF;
+                    }
+                } else {
+                    // vertical arm longer than the ring window (L > 17)
+                    for (int row = max(y - t, 0); row <= min(y + b, H - 1); row++) {
+                        const uint32_t rr2 = aref[(size_t)row * W + x], rt = atar[(size_t)row * W + tx];
+                        const int l2 = min(arm_l(rr2), arm_l(rt)), r2 = min(arm_r(rr2), arm_r(rt));
+                        const float *Pr = Pb + (size_t)row * W;
+                        sum += Pr[min(x + r2, W - 1)] - Pr[max(x - l2 - 1, 0)];
+                        cnt += l2 + r2 + 1;
+                    }
+                }
+                out[base + (size_t)y * W + x] = __fdiv_rn(sum, (float)cnt);
             }
-        } else {
-            // Vertical arms beyond the halo (only with L > 17): global memory.
-            for (int row = max(ia + rlo, 0); row <= min(ib + rlo, H - 1); row++) {
-                const uint32_t rr = aref[(size_t)row * W + x], rt = atar[(size_t)row * W + tx];
-                const int l = min(arm_l(rr), arm_l(rt)), r = min(arm_r(rr), arm_r(rt));
-                const float *Pr = P + base + (size_t)row * W;
-                sum += Pr[min(x + r, W - 1)] - Pr[max(x - l - 1, 0)];
-                cnt += l + r + 1;
-            }
-        }
-        out[base + (size_t)y * W + x] = __fdiv_rn(sum, (float)cnt);
+        out_next = max(out_next, out_end);
+        __syncthreads();
     }
 }
 
@@ -618,10 +650,10 @@ rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, con
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(P != out, "cross aggregation is out of place");
     RTS_CHECK(mode == RTS_CROSS_EXACT || mode == RTS_CROSS_FAST, "bad cross mode %d", mode);
+    const int nx = div_up(W, CV_TX), nseg = div_up(H, SW_SEG);
+    RTS_CHECK((long)nx * nseg * D < (1L << 31), "volume too large");
     // Both mode values run the same sweep, which adds in the reference's order.
-    RTS_CHECK(div_up(H, PV_TY) <= 65535, "height %d too large", H);
-    dim3 grid(div_up(W, CV_TX), div_up(H, PV_TY), D);
-    k_cross_pv<<<grid, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H);
+    k_cross_strip<<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
     RTS_LAUNCHED();
     return RTS_OK;
 }

@@ -116,43 +116,6 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
     }
 }
 
-// The same combined cost without the prefix: one thread per x over a chunk
-// of disparities, every store a contiguous row segment of plane d.  The
-// right view's census word and colour at x-d come through the caches (a
-// block's lanes read consecutive x-d).
-constexpr int FC_XB = 256;
-constexpr int FC_DCH = 16;
-
-template <bool CENSUS, bool AD>
-__global__ void __launch_bounds__(FC_XB) k_front_cost(const uint64_t *__restrict__ cL, const uint64_t *__restrict__ cR,
-                                                      const uint8_t *__restrict__ rgbL,
-                                                      const uint8_t *__restrict__ rgbR, const float *__restrict__ tabC,
-                                                      const float *__restrict__ tabA, int nA, float *__restrict__ C,
-                                                      int W, int H, int D) {
-    __shared__ float tC[65];
-    __shared__ float tA[766];
-    for (int i = threadIdx.x; i < 65; i += FC_XB) tC[i] = CENSUS ? tabC[i] : 0.0f;
-    for (int i = threadIdx.x; i < nA; i += FC_XB) tA[i] = tabA[i];
-    __syncthreads();
-    const int x = blockIdx.x * FC_XB + threadIdx.x, y = blockIdx.y, d0 = blockIdx.z * FC_DCH;
-    if (x >= W) return;
-    const size_t roff = (size_t)y * W, pl = plane(W, H);
-    const uint64_t cl = CENSUS ? cL[roff + x] : 0;
-    const uint32_t al = AD ? pack_rgb(rgbL + 3 * (roff + x)) : 0;
-    const int dend = min(D, d0 + FC_DCH);
-    for (int d = d0; d < dend; d++) {
-        const int xt = x >= d ? x - d : x - d + W;
-        float c;
-        if (CENSUS && AD)
-            c = tC[__popcll(cl ^ cR[roff + xt])] + tA[sad3(al, pack_rgb(rgbR + 3 * (roff + xt)))];
-        else if (CENSUS)
-            c = tC[__popcll(cl ^ cR[roff + xt])];
-        else
-            c = tA[sad3(al, pack_rgb(rgbR + 3 * (roff + xt)))];
-        C[(size_t)d * pl + roff + x] = c;
-    }
-}
-
 // The AD value the colour AD kernel produces for integer channel sum s.
 static float ad_value(int s, int use_trunc, float trunc, int norm) {
     const float third = (float)1 / (float)3;
@@ -430,19 +393,7 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
                            sizeof(float) * waves * 64 * (FH_CH + 1);
         RTS_CHECK(shm <= 160 * 1024, "front-end LDS %zu too large", shm);
         const int vec = (W % 4) == 0 && ((uintptr_t)cur % 16) == 0;
-        if (!d.cross_exact) {
-            // Raw combined costs (no row prefix): the direct cross pass sums them.
-            dim3 grid(div_up(W, FC_XB), H, div_up(D, FC_DCH));
-            if (census && ad)
-                k_front_cost<true, true><<<grid, FC_XB, 0, s>>>(p->census[0], p->census[1], rgb[0], rgb[1], p->tabC,
-                                                               p->tabA, p->nA, cur, W, H, D);
-            else if (census)
-                k_front_cost<true, false><<<grid, FC_XB, 0, s>>>(p->census[0], p->census[1], rgb[0], rgb[1], p->tabC,
-                                                                p->tabA, p->nA, cur, W, H, D);
-            else
-                k_front_cost<false, true><<<grid, FC_XB, 0, s>>>(p->census[0], p->census[1], rgb[0], rgb[1], p->tabC,
-                                                                p->tabA, p->nA, cur, W, H, D);
-        } else if (census && ad)
+        if (census && ad)
             k_front_hscan<true, true><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
                                                                  p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
         else if (census)
@@ -452,7 +403,7 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
             k_front_hscan<false, true><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
                                                                   p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
         RTS_LAUNCHED();
-        prefixed = d.cross_exact != 0;
+        prefixed = true;
     } else {
         RTS_MARK(p, "prep", s);
         if (int e = build_cost(p, d.cost1, rgb, gray, cur, s)) return e;
@@ -477,18 +428,12 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
             break;
         case RTS_AGGR_CROSS:
             for (int it = 0; it < d.cross_iters; it++) {
-                if (prefixed) {
-                    if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D, RTS_CROSS_EXACT, s))
-                        return e;
-                } else if (d.cross_exact) {
-                    // the reference's dataflow: serial row prefix, then the sweep
+                // the reference's dataflow: serial row prefix (fused into the
+                // front end for the first pass), then the vertical sweep
+                if (!prefixed)
                     if (int e = rts_hprefix(cur, W, H, D, s)) return e;
-                    if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D, RTS_CROSS_EXACT, s))
-                        return e;
-                } else {
-                    // one kernel per pass: segment sums straight from the costs
-                    if (int e = rts_cross_fused(cur, other, p->arms[0], p->arms[1], W, H, D, s)) return e;
-                }
+                const int mode = (d.cross_exact || d.cross_L > 17) ? RTS_CROSS_EXACT : RTS_CROSS_FAST;
+                if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D, mode, s)) return e;
                 prefixed = false;
                 std::swap(cur, other);
             }

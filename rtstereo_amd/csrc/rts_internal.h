@@ -54,6 +54,17 @@ __device__ __forceinline__ int wrapn(int v, int n) {
 // rounded once to float is, because 53 >= 2*24 + 2 bits.
 __device__ __forceinline__ float sqrt_rn(float x) { return (float)__builtin_sqrt((double)x); }
 
+// XCD-aware block order.  MI355X hands consecutive workgroups of a launch
+// to its 8 XCDs round-robin, and each XCD has its own L2.  Given the linear
+// workgroup id, return a work index such that XCD k processes one
+// contiguous range of the work sequence: neighbouring work items then share
+// an L2.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_order(int lin, int n) {
+    const int per = n >> 3, rem = n & 7;
+    const int xcd = lin & 7, j = lin >> 3;
+    return xcd < rem ? xcd * (per + 1) + j : rem * (per + 1) + (xcd - rem) * per + j;
+}
+
 // Plane stride of a disparity-major volume [D][H][W].
 __host__ __device__ __forceinline__ size_t plane(int W, int H) { return (size_t)W * (size_t)H; }
 
