@@ -285,6 +285,53 @@ rts_status rts_pipeline_stage_ms(rts_pipeline *p, float *ms, const char **names,
  * (disparity-major), for inspection. */
 const float *rts_pipeline_volume(rts_pipeline *p);
 
+/* ------------------------------------------------------------ serving */
+
+/* A server runs stereo requests of mixed size and configuration on `slots`
+ * execution slots (one HIP stream and one host worker each), so requests
+ * overlap on the device.  Admission is continuous: a queued request starts
+ * as soon as a slot is free and its device footprint (rts_pipeline_bytes
+ * plus input/output buffers) fits the byte budget; idle pooled pipelines are
+ * evicted least-recently-used to make room, and a request that does not fit
+ * does not block smaller ones queued behind it.  Requests with stream_id >= 0
+ * (camera streams) reuse a pipeline that stays resident, with its buffers,
+ * until rts_server_close_stream.  No reference counterpart: the reference
+ * driver processes one pair per run (seq/Main.cpp:47-299). */
+typedef struct rts_request_desc {
+    rts_pipeline_desc pipe;
+    int stream_id;                         /* camera stream, or -1 */
+    const uint8_t *left_rgb, *right_rgb;   /* host [H][W][3]; valid until the request completes */
+    const uint8_t *left_lab, *right_lab;   /* host Lab (adaptive weights), or NULL */
+    uint8_t *dl, *dr, *mask;               /* host [H][W] outputs (dr, mask may be NULL without lr_check) */
+} rts_request_desc;
+
+typedef struct rts_request_stats {
+    double submit_ms, start_ms, done_ms;   /* server clock: queued, started, results in host buffers */
+    size_t footprint;                      /* device bytes charged to the budget */
+    int status;                            /* rts_status of the request */
+} rts_request_stats;
+
+typedef struct rts_server_stats {
+    size_t budget_bytes, in_use_bytes;
+    int queued, running, resident;
+    uint64_t completed;
+} rts_server_stats;
+
+typedef struct rts_server rts_server;
+rts_status rts_server_create(size_t budget_bytes, int slots, rts_server **out);
+rts_status rts_server_destroy(rts_server *s);     /* cancels queued requests, waits for running ones */
+/* RTS_ERR_BUDGET if the request alone exceeds the budget. */
+rts_status rts_server_submit(rts_server *s, const rts_request_desc *req, uint64_t *ticket);
+/* RTS_ERR_NOTREADY until done; then the request's status (stats filled). */
+rts_status rts_server_poll(rts_server *s, uint64_t ticket, rts_request_stats *st);
+rts_status rts_server_wait(rts_server *s, uint64_t ticket, rts_request_stats *st);
+/* Cancels a request that has not started (it then reports RTS_ERR_CANCELLED). */
+rts_status rts_server_cancel(rts_server *s, uint64_t ticket);
+/* Stops or resumes admission (queued requests wait while paused). */
+rts_status rts_server_pause(rts_server *s, int paused);
+rts_status rts_server_close_stream(rts_server *s, int stream_id);
+rts_status rts_server_counters(rts_server *s, rts_server_stats *st);
+
 /* Gray conversion used by the pipelines (the reference gets gray from
  * OpenCV's imread, seq/Main.cpp:72-73): gray = (29B + 150G + 77R + 128) >> 8
  * on interleaved B,G,R bytes. */

@@ -1,0 +1,410 @@
+// Serving loop: continuous admission of stereo requests of mixed size and
+// configuration onto a fixed number of execution slots (one HIP stream and
+// one host worker each), with admission by device-memory footprint against a
+// byte budget.  Requests tagged with a camera stream id reuse a pipeline
+// (volumes, per-pixel buffers, pinned staging) that stays resident until
+// the stream is closed; one-off requests draw pipelines from a pool keyed
+// by configuration.  Work from different slots overlaps on the device.
+//
+// The reference has no counterpart: its driver processes one image pair per
+// process run (seq/Main.cpp:47-299).
+#include "rts_internal.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double now_ms(clk::time_point t0) {
+    return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+}
+
+bool same_desc(const rts_pipeline_desc &a, const rts_pipeline_desc &b) {
+    return memcmp(&a, &b, sizeof(a)) == 0;
+}
+
+struct Resident {
+    rts_pipeline_desc desc{};
+    rts_pipeline *pipe = nullptr;
+    uint8_t *dev = nullptr;    // device inputs (rgb x2, lab x2) and outputs (dl, dr, mask)
+    uint8_t *host = nullptr;   // pinned staging of the same layout
+    size_t dev_bytes = 0, footprint = 0;
+    bool busy = false;
+    int stream_id = -1;        // camera stream owning it, or -1 for the pool
+    uint64_t last_use = 0;
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    rts_request_desc req{};
+    size_t footprint = 0;
+    rts_request_stats st{};
+    bool done = false, started = false;
+};
+
+size_t io_bytes(const rts_pipeline_desc &d) {
+    const size_t P = (size_t)d.width * d.height;
+    return 2 * 3 * P + 2 * 3 * P + 3 * P;   // rgb, lab, dl/dr/mask
+}
+
+}  // namespace
+
+struct rts_server {
+    size_t budget = 0, in_use = 0;
+    int nslots = 0;
+    bool stop = false, paused = false;
+    clk::time_point t0 = clk::now();
+    uint64_t next_ticket = 1, use_clock = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::map<uint64_t, std::shared_ptr<Request>> all;
+    std::vector<std::unique_ptr<Resident>> residents;
+    std::vector<std::shared_ptr<Request>> assigned;   // per slot
+    std::vector<Resident *> assigned_res;
+    std::vector<std::thread> workers;
+    std::thread dispatcher;
+    uint64_t completed = 0;
+};
+
+namespace {
+
+// Footprint a request adds to the budget: a new pipeline and its buffers,
+// unless a resident one can serve it.
+size_t pipeline_footprint(const rts_pipeline_desc &d) { return rts_pipeline_bytes(&d) + io_bytes(d); }
+
+// An idle resident pipeline that can serve `r`: the camera stream's own one,
+// or one from the pool with the same configuration.
+Resident *find_resident(rts_server *s, const Request &r) {
+    for (auto &res : s->residents) {
+        if (res->busy) continue;
+        if (r.req.stream_id >= 0) {
+            if (res->stream_id == r.req.stream_id && same_desc(res->desc, r.req.pipe)) return res.get();
+        } else if (res->stream_id < 0 && same_desc(res->desc, r.req.pipe)) {
+            return res.get();
+        }
+    }
+    return nullptr;
+}
+
+// A camera stream's frames run one at a time, in order.
+bool stream_busy(const rts_server *s, int stream_id) {
+    if (stream_id < 0) return false;
+    for (auto &res : s->residents)
+        if (res->busy && res->stream_id == stream_id) return true;
+    return false;
+}
+
+// Bytes that evicting every idle pipeline would release.
+size_t idle_bytes(const rts_server *s) {
+    size_t n = 0;
+    for (auto &res : s->residents)
+        if (!res->busy) n += res->footprint;
+    return n;
+}
+
+// Frees idle pipelines until `need` fits: pool pipelines first, then idle
+// camera pipelines (which are rebuilt on the stream's next frame), least
+// recently used first within each group.
+void evict_for(rts_server *s, size_t need) {
+    while (s->in_use + need > s->budget) {
+        Resident *victim = nullptr;
+        for (auto &res : s->residents) {
+            if (res->busy) continue;
+            if (!victim) {
+                victim = res.get();
+                continue;
+            }
+            const bool pool = res->stream_id < 0, vpool = victim->stream_id < 0;
+            if ((pool && !vpool) || (pool == vpool && res->last_use < victim->last_use)) victim = res.get();
+        }
+        if (!victim) return;
+        s->in_use -= victim->footprint;
+        rts_pipeline_destroy(victim->pipe);
+        (void)hipFree(victim->dev);
+        (void)hipHostFree(victim->host);
+        s->residents.erase(std::find_if(s->residents.begin(), s->residents.end(),
+                                        [&](const std::unique_ptr<Resident> &p) { return p.get() == victim; }));
+    }
+}
+
+int make_resident(rts_server *s, const rts_pipeline_desc &d, int stream_id, Resident **out) {
+    if (stream_id >= 0)   // a camera that changed configuration: its old pipeline joins the pool
+        for (auto &res : s->residents)
+            if (res->stream_id == stream_id) res->stream_id = -1;
+    std::unique_ptr<Resident> res(new Resident());
+    res->desc = d;
+    res->stream_id = stream_id;
+    res->dev_bytes = io_bytes(d);
+    if (int e = rts_pipeline_create(&d, &res->pipe)) return e;
+    if (hipMalloc((void **)&res->dev, res->dev_bytes) != hipSuccess ||
+        hipHostMalloc((void **)&res->host, res->dev_bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        rts_pipeline_destroy(res->pipe);
+        (void)hipFree(res->dev);
+        return rts::fail(RTS_ERR_NOMEM, "server: staging allocation of %zu bytes failed", res->dev_bytes);
+    }
+    res->footprint = pipeline_footprint(d);
+    *out = res.get();
+    s->residents.push_back(std::move(res));
+    return RTS_OK;
+}
+
+// One request on one slot: stage inputs, run, stage outputs.
+int execute(Resident *res, const rts_request_desc &q, hipStream_t st) {
+    const rts_pipeline_desc &d = res->desc;
+    const size_t P = (size_t)d.width * d.height;
+    uint8_t *h = res->host, *g = res->dev;
+    const size_t oL = 0, oR = 3 * P, oLabL = 6 * P, oLabR = 9 * P, oDL = 12 * P, oDR = 13 * P, oM = 14 * P;
+    memcpy(h + oL, q.left_rgb, 3 * P);
+    memcpy(h + oR, q.right_rgb, 3 * P);
+    const bool lab = q.left_lab && q.right_lab;
+    if (lab) {
+        memcpy(h + oLabL, q.left_lab, 3 * P);
+        memcpy(h + oLabR, q.right_lab, 3 * P);
+    }
+    RTS_HIP(hipMemcpyAsync(g, h, (lab ? 12 : 6) * P, hipMemcpyHostToDevice, st));
+    if (int e = rts_pipeline_run(res->pipe, g + oL, g + oR, nullptr, nullptr, lab ? g + oLabL : nullptr,
+                                 lab ? g + oLabR : nullptr, g + oDL, g + oDR, g + oM, st))
+        return e;
+    RTS_HIP(hipMemcpyAsync(h + oDL, g + oDL, 3 * P, hipMemcpyDeviceToHost, st));
+    RTS_HIP(hipStreamSynchronize(st));
+    memcpy(q.dl, h + oDL, P);
+    if (q.dr) memcpy(q.dr, h + oDR, P);
+    if (q.mask) memcpy(q.mask, h + oM, P);
+    return RTS_OK;
+}
+
+void worker_main(rts_server *s, int slot) {
+    hipStream_t st = nullptr;
+    (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_work.wait(lk, [&] { return s->stop || s->assigned[slot] != nullptr; });
+        if (s->stop && !s->assigned[slot]) break;
+        std::shared_ptr<Request> r = s->assigned[slot];
+        Resident *res = s->assigned_res[slot];
+        lk.unlock();
+        r->st.start_ms = now_ms(s->t0);
+        int e = st ? execute(res, r->req, st) : rts::fail(RTS_ERR_HIP, "server: stream creation failed");
+        const double done = now_ms(s->t0);
+        lk.lock();
+        r->st.done_ms = done;
+        r->st.status = e;
+        r->done = true;
+        res->busy = false;
+        res->last_use = ++s->use_clock;
+        s->assigned[slot] = nullptr;
+        s->assigned_res[slot] = nullptr;
+        s->completed++;
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();   // wake the dispatcher: a slot and maybe memory are free
+    }
+    lk.unlock();
+    if (st) (void)hipStreamDestroy(st);
+}
+
+// Admission: requests in arrival order.  A request starts when a slot is
+// free and it either finds an idle resident pipeline of its configuration or
+// its footprint fits the budget once idle pipelines are evicted.  Once one
+// request is waiting for memory, the requests behind it may still start on
+// resident pipelines but may not allocate, so a large request is not starved
+// by a stream of smaller ones.
+void dispatcher_main(rts_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->stop) {
+        bool progressed = false;
+        if (!s->paused) {
+            bool blocked = false;
+            for (auto it = s->queue.begin(); it != s->queue.end();) {
+                int slot = -1;
+                for (int i = 0; i < s->nslots; i++)
+                    if (!s->assigned[i]) {
+                        slot = i;
+                        break;
+                    }
+                if (slot < 0) break;
+                std::shared_ptr<Request> r = *it;
+                if (stream_busy(s, r->req.stream_id)) {
+                    ++it;
+                    continue;
+                }
+                Resident *res = find_resident(s, *r);
+                if (!res) {
+                    if (blocked || s->in_use - idle_bytes(s) + r->footprint > s->budget) {
+                        blocked = true;
+                        ++it;
+                        continue;
+                    }
+                    evict_for(s, r->footprint);
+                    if (int e = make_resident(s, r->req.pipe, r->req.stream_id, &res)) {
+                        r->st.status = e;
+                        r->done = true;
+                        it = s->queue.erase(it);
+                        s->cv_done.notify_all();
+                        continue;
+                    }
+                    s->in_use += res->footprint;
+                }
+                res->busy = true;
+                r->started = true;
+                r->st.footprint = res->footprint;
+                s->assigned[slot] = r;
+                s->assigned_res[slot] = res;
+                it = s->queue.erase(it);
+                progressed = true;
+            }
+        }
+        if (progressed) s->cv_work.notify_all();
+        s->cv_work.wait(lk);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+rts_status rts_server_create(size_t budget_bytes, int slots, rts_server **out) {
+    RTS_CHECK(out != nullptr, "null out pointer");
+    RTS_CHECK(slots >= 1 && slots <= 16, "slots %d must be in [1, 16]", slots);
+    RTS_CHECK(budget_bytes > 0, "empty budget");
+    rts_server *s = new rts_server();
+    s->budget = budget_bytes;
+    s->nslots = slots;
+    s->assigned.assign(slots, nullptr);
+    s->assigned_res.assign(slots, nullptr);
+    for (int i = 0; i < slots; i++) s->workers.emplace_back(worker_main, s, i);
+    s->dispatcher = std::thread(dispatcher_main, s);
+    *out = s;
+    return RTS_OK;
+}
+
+rts_status rts_server_destroy(rts_server *s) {
+    if (!s) return RTS_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+        for (auto &r : s->queue) {
+            r->st.status = RTS_ERR_CANCELLED;
+            r->done = true;
+        }
+        s->queue.clear();
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    s->dispatcher.join();
+    for (auto &t : s->workers) t.join();
+    for (auto &res : s->residents) {
+        rts_pipeline_destroy(res->pipe);
+        (void)hipFree(res->dev);
+        (void)hipHostFree(res->host);
+    }
+    delete s;
+    return RTS_OK;
+}
+
+rts_status rts_server_submit(rts_server *s, const rts_request_desc *req, uint64_t *ticket) {
+    RTS_CHECK(s && req && ticket, "null argument");
+    RTS_CHECK(req->left_rgb && req->right_rgb && req->dl, "request needs both colour images and dl");
+    RTS_CHECK(!req->pipe.lr_check || (req->dr && req->mask), "lr_check needs dr and mask");
+    RTS_CHECK(req->pipe.aggr != RTS_AGGR_AW || (req->left_lab && req->right_lab), "AW requests need Lab images");
+    if (int e = rts::check_dims(req->pipe.width, req->pipe.height, req->pipe.disp)) return e;
+    auto r = std::make_shared<Request>();
+    r->req = *req;
+    r->footprint = pipeline_footprint(req->pipe);
+    std::lock_guard<std::mutex> g(s->mu);
+    if (r->footprint > s->budget)
+        return rts::fail(RTS_ERR_BUDGET, "request footprint %zu bytes exceeds the budget of %zu bytes", r->footprint,
+                         s->budget);
+    r->ticket = s->next_ticket++;
+    r->st.submit_ms = now_ms(s->t0);
+    r->st.footprint = r->footprint;
+    s->queue.push_back(r);
+    s->all[r->ticket] = r;
+    *ticket = r->ticket;
+    s->cv_work.notify_all();
+    return RTS_OK;
+}
+
+static rts_status finish(rts_server *s, uint64_t ticket, rts_request_stats *st, bool wait) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    RTS_CHECK(it != s->all.end(), "unknown ticket %llu", (unsigned long long)ticket);
+    std::shared_ptr<Request> r = it->second;
+    if (wait) s->cv_done.wait(lk, [&] { return r->done; });
+    if (!r->done) return RTS_ERR_NOTREADY;
+    if (st) *st = r->st;
+    s->all.erase(it);
+    return r->st.status;
+}
+
+rts_status rts_server_poll(rts_server *s, uint64_t ticket, rts_request_stats *st) {
+    RTS_CHECK(s != nullptr, "null server");
+    return finish(s, ticket, st, false);
+}
+
+rts_status rts_server_wait(rts_server *s, uint64_t ticket, rts_request_stats *st) {
+    RTS_CHECK(s != nullptr, "null server");
+    return finish(s, ticket, st, true);
+}
+
+rts_status rts_server_cancel(rts_server *s, uint64_t ticket) {
+    RTS_CHECK(s != nullptr, "null server");
+    std::lock_guard<std::mutex> g(s->mu);
+    for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+        if ((*it)->ticket == ticket) {
+            (*it)->st.status = RTS_ERR_CANCELLED;
+            (*it)->done = true;
+            s->queue.erase(it);
+            s->cv_done.notify_all();
+            return RTS_OK;
+        }
+    return rts::fail(RTS_ERR_INVALID, "ticket %llu is not waiting (running, finished or unknown)",
+                     (unsigned long long)ticket);
+}
+
+rts_status rts_server_pause(rts_server *s, int paused) {
+    RTS_CHECK(s != nullptr, "null server");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->paused = paused != 0;
+    }
+    s->cv_work.notify_all();
+    return RTS_OK;
+}
+
+rts_status rts_server_close_stream(rts_server *s, int stream_id) {
+    RTS_CHECK(s != nullptr, "null server");
+    std::lock_guard<std::mutex> g(s->mu);
+    for (auto &res : s->residents)
+        if (res->stream_id == stream_id && !res->busy) res->stream_id = -1;   // back to the pool
+    s->cv_work.notify_all();
+    return RTS_OK;
+}
+
+rts_status rts_server_counters(rts_server *s, rts_server_stats *c) {
+    RTS_CHECK(s && c, "null argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    c->budget_bytes = s->budget;
+    c->in_use_bytes = s->in_use;
+    c->queued = (int)s->queue.size();
+    int running = 0;
+    for (auto &a : s->assigned) running += a != nullptr;
+    c->running = running;
+    c->resident = (int)s->residents.size();
+    c->completed = s->completed;
+    return RTS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+"""Serving loop on the GPU: a request's result does not depend on what runs
+beside it, camera streams keep their pipeline resident, and a budget smaller
+than the offered load queues and evicts instead of failing."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from rtstereo_amd import gen
+from rtstereo_amd import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+from rtstereo_amd import pipeline as PL  # noqa: E402
+from rtstereo_amd import serve  # noqa: E402
+
+
+def _alone(desc, p, lab=None):
+    dl, dr, m, _ = PL.run_host(desc, p.left, p.right, *(lab or (None, None)))
+    return dl, dr, m
+
+
+def _mix(seed=21):
+    cams = [gen.make_pair(192, 128, 32, 8, 2.5, seed + c) for c in range(3)]
+    bursts = [gen.make_pair(160, 96, 24, 6, 2.5, seed + 10), gen.make_pair(224, 112, 40, 6, 2.5, seed + 11)]
+    return cams, bursts
+
+
+def test_results_independent_of_mix():
+    print("[case] server mix 3 cameras x 3 frames + 2 AW bursts (fp32, fp16) + 1 plain", flush=True)
+    cams, bursts = _mix()
+    cdesc = PL.preset("census_cross", 192, 128, 32)
+    exp_cam = [_alone(cdesc, p) for p in cams]
+    blabs = [b.lab() for b in bursts]
+    bdesc = [PL.preset("aw", 160, 96, 24, aw_precision=L.RTS_AW_FP32),
+             PL.preset("aw", 224, 112, 40, aw_precision=L.RTS_AW_FP16)]
+    exp_b = [_alone(d, b, lab) for d, b, lab in zip(bdesc, bursts, blabs)]
+    pdesc = PL.preset("plain", 160, 96, 24)
+    exp_p = _alone(pdesc, bursts[0])
+
+    s = serve.Server(budget_bytes=8 << 30, slots=4)
+    got_cam, got_b = [], []
+    for f in range(3):
+        for c, p in enumerate(cams):
+            got_cam.append((c, s.submit(cdesc, p.left, p.right, stream_id=c, kind="camera")))
+        if f == 1:
+            got_b = [s.submit(d, b.left, b.right, *lab, kind="burst") for d, b, lab in zip(bdesc, bursts, blabs)]
+            got_p = s.submit(pdesc, bursts[0].left, bursts[0].right, kind="plain")
+    for c, t in got_cam:
+        s.wait(t)
+        assert np.array_equal(t.dl, exp_cam[c][0]) and np.array_equal(t.dr, exp_cam[c][1])
+        assert np.array_equal(t.mask, exp_cam[c][2])
+        assert t.stats.start_ms >= t.stats.submit_ms and t.stats.done_ms >= t.stats.start_ms
+    for t, e in zip(got_b, exp_b):
+        s.wait(t)
+        assert np.array_equal(t.dl, e[0]) and np.array_equal(t.mask, e[2])
+    s.wait(got_p)
+    assert np.array_equal(got_p.dl, exp_p[0])
+    c = s.counters()
+    assert c.completed == 12 and c.queued == 0 and c.running == 0
+    assert c.resident <= 3 + 3 * 4     # camera pipelines stay, pool ones are reused or kept
+    s.close()
+
+
+def test_small_budget_queues_and_evicts():
+    """Budget of 1.5 footprints: requests of three configurations must run
+    one at a time, evicting each other's idle pipelines, and still agree."""
+    print("[case] server budget 1.5 footprints, 3 configurations x 2", flush=True)
+    cams, bursts = _mix(31)
+    descs = [PL.preset("census_cross", 192, 128, 32), PL.preset("census_cross", 192, 128, 24),
+             PL.preset("plain", 192, 128, 32)]
+    need = max(int(L.lib.rts_pipeline_bytes(C.byref(d))) + 15 * 192 * 128 for d in descs)
+    s = serve.Server(budget_bytes=int(need * 1.5), slots=4)
+    exp = [_alone(d, cams[i]) for i, d in enumerate(descs)]
+    tickets = []
+    for rep in range(2):
+        for i, d in enumerate(descs):
+            tickets.append((i, s.submit(d, cams[i].left, cams[i].right, stream_id=(i if rep == 0 else -1))))
+    for i, t in tickets:
+        s.wait(t)
+        assert np.array_equal(t.dl, exp[i][0]) and np.array_equal(t.mask, exp[i][2])
+    c = s.counters()
+    assert c.in_use_bytes <= c.budget_bytes
+    assert 1 <= c.resident <= 2
+    s.close()
+
+
+def test_camera_stream_frames_in_order_and_resident():
+    print("[case] server one camera, 6 frames", flush=True)
+    frames = [gen.make_pair(128, 96, 16, 6, 2.5, 40 + f) for f in range(6)]
+    d = PL.preset("census_cross", 128, 96, 16)
+    s = serve.Server(budget_bytes=1 << 30, slots=4)
+    ts = [s.submit(d, p.left, p.right, stream_id=7) for p in frames]
+    for t in ts:
+        s.wait(t)
+    for a, b in zip(ts, ts[1:]):
+        assert b.stats.start_ms >= a.stats.done_ms - 1e-6        # one frame at a time, in order
+    assert s.counters().resident == 1
+    for p, t in zip(frames, ts):
+        assert np.array_equal(t.dl, _alone(d, p)[0])
+    s.close_stream(7)
+    t = s.submit(d, frames[0].left, frames[0].right)              # the closed stream's pipeline joins the pool
+    s.wait(t)
+    assert s.counters().resident == 1
+    s.close()
