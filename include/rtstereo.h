@@ -164,13 +164,22 @@ rts_status rts_compute_weights(float *weights, const float *prox, const uint8_t 
 rts_status rts_aw_weights(const float *wref, const float *wtar, const float *vol, float *out, int W, int H, int D,
                           int r, int dir, void *stream);
 
-/* Adaptive-weight aggregation with the weights built on the fly from the two
- * Lab images (no weight tensors).  precision: RTS_AW_FP32 reproduces the
- * reference's float arithmetic; RTS_AW_FP16 uses half weights and products
- * with float accumulation. */
+/* Adaptive-weight aggregation from the two Lab images, without the
+ * reference's whole-image weight tensors: seq/AdaptiveWeightAggregation.cpp
+ * :11-18 calcProxWeight, :33-71 computeWeights and :73-111 calcAWCostL2R in
+ * one call.  The weights of a band of rows go to a workspace, then the band
+ * is aggregated.  precision: RTS_AW_FP32 float weights, products and sums
+ * (within 1e-5 relative of the FP64 result); RTS_AW_FP16 half weights and
+ * products with float accumulation.  r <= 16.  rts_aw allocates the
+ * workspace on the stream; rts_aw_ws takes one of at least
+ * rts_aw_workspace_bytes(W, H, r, precision) bytes. */
 enum { RTS_AW_FP32 = 0, RTS_AW_FP16 = 1 };
 rts_status rts_aw(const uint8_t *lab_ref, const uint8_t *lab_tar, const float *vol, float *out, int W, int H, int D,
                   int r, float gamma_c, float gamma_p, int precision, void *stream);
+size_t rts_aw_workspace_bytes(int W, int H, int r, int precision);
+rts_status rts_aw_ws(const uint8_t *lab_ref, const uint8_t *lab_tar, const float *vol, float *out, int W, int H,
+                     int D, int r, float gamma_c, float gamma_p, int precision, void *workspace,
+                     size_t workspace_bytes, void *stream);
 
 /* Cross arms (left, right, top, bottom), at most L, colour threshold tao.
  * seq/CrossBasedAggregation.cpp:12-115 calc_x_range / calc_y_range / findCross. */

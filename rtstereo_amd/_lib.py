@@ -77,6 +77,7 @@ _SIGS = {
     "rts_compute_weights": [vp, vp, vp, f32, i32, i32, i32, vp],
     "rts_aw_weights": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],
     "rts_aw": [vp, vp, vp, vp, i32, i32, i32, i32, f32, f32, i32, vp],
+    "rts_aw_ws": [vp, vp, vp, vp, i32, i32, i32, i32, f32, f32, i32, vp, sz, vp],
     "rts_find_cross": [vp, vp, i32, i32, i32, i32, vp],
     "rts_arms_unpack": [vp, vp, vp, i32, i32, vp], "rts_arms_pack": [vp, vp, vp, i32, i32, vp],
     "rts_hprefix": [vp, i32, i32, i32, vp],
@@ -87,6 +88,7 @@ _SIGS = {
     "rts_disp_refine": [vp, vp, vp, i32, i32, i32, i32, vp],
     "rts_pipeline_defaults": [C.POINTER(PipelineDesc), i32, i32, i32],
     "rts_pipeline_bytes": [C.POINTER(PipelineDesc)],
+    "rts_aw_workspace_bytes": [i32, i32, i32, i32],
     "rts_pipeline_create": [C.POINTER(PipelineDesc), C.POINTER(vp)],
     "rts_pipeline_destroy": [vp],
     "rts_pipeline_run": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
@@ -96,7 +98,8 @@ _SIGS = {
     "rts_rgb_to_gray": [vp, vp, i32, i32, vp],
 }
 _RESTYPES = {"rts_last_error": cp, "rts_version": cp, "rts_pipeline_defaults": None,
-             "rts_pipeline_bytes": sz, "rts_pipeline_volume": vp}
+             "rts_pipeline_bytes": sz, "rts_pipeline_volume": vp,
+             "rts_aw_workspace_bytes": sz}
 
 for _name, _args in _SIGS.items():
     if not hasattr(lib, _name):

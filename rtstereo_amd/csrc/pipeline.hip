@@ -143,6 +143,8 @@ struct rts_pipeline {
     // volumes
     float *vol[2] = {nullptr, nullptr};
     float *final_vol = nullptr;
+    void *aw_ws = nullptr;   // adaptive-weight band workspace
+    size_t aw_ws_bytes = 0;
     // tables for the fused front end
     float *tabC = nullptr, *tabA = nullptr;
     int nA = 0;
@@ -245,6 +247,7 @@ size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
     if (d->aggr == RTS_AGGR_CROSS) b += 2 * P * 4;
     if (uses(*d, RTS_COST_RANK)) b += 2 * P;
     if (uses(*d, RTS_COST_LOG)) b += 2 * P * 4;
+    if (d->aggr == RTS_AGGR_AW) b += rts_aw_workspace_bytes(d->width, d->height, d->aw_r, d->aw_precision);
     return b + 8192;
 }
 
@@ -269,6 +272,10 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
         if (uses(d, RTS_COST_RANK)) A((void **)&p->rank[v], P);
         if (uses(d, RTS_COST_LOG)) A((void **)&p->logimg[v], P * 4);
         A((void **)&p->vol[v], V);
+    }
+    if (d.aggr == RTS_AGGR_AW) {
+        p->aw_ws_bytes = rts_aw_workspace_bytes(d.width, d.height, d.aw_r, d.aw_precision);
+        A(&p->aw_ws, p->aw_ws_bytes);
     }
     // Fused front end: census and/or colour AD (combined) feeding cross-based aggregation.
     const bool front_ok = d.aggr == RTS_AGGR_CROSS && d.width >= 8 &&
@@ -312,6 +319,7 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
         (void)hipFree(p->logimg[v]);
         (void)hipFree(p->vol[v]);
     }
+    (void)hipFree(p->aw_ws);
     (void)hipFree(p->tabC);
     (void)hipFree(p->tabA);
     for (int i = 0; i < 8; i++)
@@ -421,8 +429,8 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
             std::swap(cur, other);
             break;
         case RTS_AGGR_AW:
-            if (int e = rts_aw(left_lab, right_lab, cur, other, W, H, D, d.aw_r, d.aw_gamma_c, d.aw_gamma_p,
-                               d.aw_precision, s))
+            if (int e = rts_aw_ws(left_lab, right_lab, cur, other, W, H, D, d.aw_r, d.aw_gamma_c, d.aw_gamma_p,
+                                  d.aw_precision, p->aw_ws, p->aw_ws_bytes, s))
                 return e;
             std::swap(cur, other);
             break;

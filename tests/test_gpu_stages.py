@@ -148,6 +148,43 @@ def test_aw_on_the_fly(W, H, D, seed, r):
     assert differ[margin > 2 * err16].mean() <= 1e-3 if (margin > 2 * err16).any() else True
 
 
+@pytest.mark.parametrize("r", [1, 5, 8, 10])
+def test_aw_padded_radius(r):
+    """Radii below the kernel's compiled window run with zero-padded taps."""
+    W, H, D, seed = 67, 41, 13, 9
+    p = _pair(W, H, D, seed)
+    ll, lr = p.lab()
+    c = oracle.ad_color(p.left, p.right, D, 1, 40.0, False)
+    ys, xs = np.mgrid[0:H, 0:W]
+    f64 = oracle.aw_l2r_f64(ll, lr, c, ys.ravel(), xs.ravel(), r, 7.0, 36.0).reshape(H, W, D)
+    got32 = pm(S.aw(ll, lr, dm(c), r, 7.0, 36.0, L.RTS_AW_FP32)).astype(np.float64)
+    assert np.max(np.abs(got32 - f64) / np.maximum(np.abs(f64), 1e-30)) <= 1e-5
+    got16 = pm(S.aw(ll, lr, dm(c), r, 7.0, 36.0, L.RTS_AW_FP16)).astype(np.float64)
+    assert np.max(np.abs(got16 - f64)) < 0.05
+
+
+def test_aw_bands_full_width():
+    """1920 wide, 200 rows, r=16: the weight workspace holds 64 rows of the
+    FP32 form, so the image runs in four bands; pixels on and beside the
+    band edges and the image borders match the FP64 result."""
+    W, H, D, r = 1920, 200, 64, 16
+    p = gen.make_pair(W, H, D, 20, 3.0, 12)
+    print(f"[case] aw bands {W}x{H} D={D} r={r}", flush=True)
+    ll, lr = p.lab()
+    c = oracle.ad_color(p.left, p.right, D, 1, 40.0, False)
+    rng = np.random.default_rng(3)
+    rows = np.array([0, 1, 62, 63, 64, 65, 127, 128, 191, 192, 198, 199])
+    cols = np.concatenate([[0, 1, 63, 64, W - 2, W - 1], rng.integers(0, W, 14)])
+    ys, xs = (a.ravel() for a in np.meshgrid(rows, cols, indexing="ij"))
+    f64 = oracle.aw_l2r_f64(ll, lr, c, ys, xs, r, 7.0, 36.0).reshape(len(ys), D)
+    for prec in (L.RTS_AW_FP32, L.RTS_AW_FP16):
+        got = S.aw(ll, lr, dm(c), r, 7.0, 36.0, prec)[:, ys, xs].T.astype(np.float64)
+        if prec == L.RTS_AW_FP32:
+            assert np.max(np.abs(got - f64) / np.maximum(np.abs(f64), 1e-30)) <= 1e-5
+        else:
+            assert np.max(np.abs(got - f64)) < 0.05
+
+
 @pytest.mark.parametrize("W,H,D,seed", SIZES)
 def test_cross(W, H, D, seed):
     p = _pair(W, H, D, seed)
