@@ -31,7 +31,7 @@ namespace rts {
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 constexpr int AW_RMAX = 16;               // largest window radius
-constexpr int AW_TX = 64;                 // pixels per tile
+constexpr int AW_TX = 64;                 // weight rows are padded to a multiple of this
 constexpr int AW_NX = 8;                  // pixels per lane
 constexpr int AW_NY = 2;                  // rows per tile
 constexpr size_t AW_WS_CAP = size_t(1) << 30;   // workspace bytes for both views
@@ -41,29 +41,35 @@ __host__ __device__ inline int aw_ntap(int R, bool half) { return half ? R + 1 :
 
 __host__ inline int aw_rmax_for(int r) { return r <= 3 ? 3 : r <= 5 ? 5 : r <= 8 ? 8 : 16; }
 
-// Weights of band rows [y0, y0+rows) of one view: wb[ry][dy+r][t][x], x < Wp.
-// Block: 64 pixels of one (row, window row); the four waves split the taps.
+// Weights of band rows [y0, y0+rows) of one view: wb[ry][dy+r][x][PG], the
+// NT words of a pixel's window row contiguous and padded to PG (a multiple
+// of 4) so that a row is whole 16-byte words.  Block: 64 pixels x 4 window
+// rows; each thread computes all taps of one (pixel, window row).
 // prox(dy, dx) = exp(-sqrt(dy^2+dx^2)/gp) comes from a per-block table; the
 // colour term is exp2(|dLab| * kc) with kc = -log2(e)/gc (hardware exp2 and
 // square root, each within an ulp or two: relative weight error ~1e-6).
 template <bool HALF, int R>
 __global__ void __launch_bounds__(256) k_aw_weights_band(const uint8_t *__restrict__ lab, uint32_t *__restrict__ wb,
-                                                         int W, int H, int Wp, int y0, int r, float kc, float gp) {
-    constexpr int NT = HALF ? R + 1 : 2 * R + 1;
-    __shared__ float prox[2 * R + 2];
+                                                         int W, int H, int Wp, int y0, int nrow, int r, float kc,
+                                                         float gp) {
+    constexpr int NT = HALF ? R + 1 : 2 * R + 1, PG = (NT + 3) & ~3;
+    __shared__ float prox[4][2 * R + 2];
     const int e = 2 * r + 1;
-    const int row = blockIdx.y, ry = row / e, dy = row - ry * e - r;
+    const int row = blockIdx.y * 4 + (threadIdx.x >> 6);   // ry * e + (dy + r)
+    const int ry = row / e, dy = row - ry * e - r;
     const int y = y0 + ry, yy = y + dy;
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    if (threadIdx.x < 2 * R + 2) {
-        const int dx = (int)threadIdx.x - R;
-        prox[threadIdx.x] =
+    if ((threadIdx.x & 63) < 2 * R + 2) {
+        const int dx = (int)(threadIdx.x & 63) - R;
+        prox[threadIdx.x >> 6][threadIdx.x & 63] =
             (dx > r || dx < -r || dy > r || dy < -r) ? 0.0f : expf(-sqrtf((float)(dy * dy + dx * dx)) / gp);
     }
     __syncthreads();
+    if (row >= nrow) return;
+    const float *pr = prox[threadIdx.x >> 6];
     const bool ok = x < W && yy >= 0 && yy < H;
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-    const uint8_t *nrow = lab + 3 * (size_t)(ok ? yy : 0) * W;
+    const uint8_t *nrowp = lab + 3 * (size_t)(ok ? yy : 0) * W;
     if (ok) {
         const uint8_t *c = lab + 3 * ((size_t)y * W + x);
         c0 = c[0];
@@ -72,24 +78,40 @@ __global__ void __launch_bounds__(256) k_aw_weights_band(const uint8_t *__restri
     }
     auto weight = [&](int dx) {
         const int xx = x + dx;
-        if (!ok || xx < 0 || xx >= W) return 0.0f;
-        const uint8_t *t = nrow + 3 * xx;
+        if (!ok || dx > R || xx < 0 || xx >= W) return 0.0f;
+        const uint8_t *t = nrowp + 3 * xx;
         const float e0 = c0 - (float)t[0], e1 = c1 - (float)t[1], e2 = c2 - (float)t[2];
         const float dist = __builtin_amdgcn_sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
-        return prox[dx + R] * __builtin_amdgcn_exp2f(dist * kc);
+        return pr[dx + R] * __builtin_amdgcn_exp2f(dist * kc);
     };
-    uint32_t *dst = wb + (size_t)row * NT * Wp + x;
-    for (int jt = threadIdx.x >> 6; jt < NT; jt += 4) {
-        if (HALF) {
-            const int dx = 2 * jt - R;
-            h2 v;
-            v.x = (_Float16)weight(dx);
-            v.y = (_Float16)weight(dx + 1);
-            dst[(size_t)jt * Wp] = __builtin_bit_cast(uint32_t, v);
-        } else {
-            dst[(size_t)jt * Wp] = __float_as_uint(weight(jt - R));
+    uint4 *dst = reinterpret_cast<uint4 *>(wb + ((size_t)row * Wp + x) * PG);
+#pragma unroll
+    for (int g = 0; g < PG / 4; g++) {
+        uint32_t w4[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int jt = 4 * g + i;
+            if (jt >= NT) {
+                w4[i] = 0u;
+            } else if (HALF) {
+                h2 v;
+                v.x = (_Float16)weight(2 * jt - R);
+                v.y = (_Float16)weight(2 * jt - R + 1);
+                w4[i] = __builtin_bit_cast(uint32_t, v);
+            } else {
+                w4[i] = __float_as_uint(weight(jt - R));
+            }
         }
+        dst[g] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
     }
+}
+
+// The value, hidden from constant folding (an empty asm that claims to
+// modify it).  Keeps the compiler from re-deriving a per-access base
+// address from a folded constant.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
 }
 
 __device__ __forceinline__ int wrap_any(int v, int n) {
@@ -98,40 +120,43 @@ __device__ __forceinline__ int wrap_any(int v, int n) {
     return (v < 0 || v >= n) ? wrapn(v, n) : v;
 }
 
-// LDS layout and staging shape of one aggregation tile.
+// LDS layout of one aggregation tile: 64 pixels x 2 rows x 64 disparities,
+// 8 waves.
 template <bool HALF, int R>
 struct AwTile {
-    static constexpr int NT = HALF ? R + 1 : 2 * R + 1;   // weight words per (pixel, window row)
-    // Weight rows padded to 16 bytes and read with ds_read_b128.  A pitch of
-    // 20 or 36 words puts 16 consecutive lanes (consecutive centres) on 16
-    // distinct 4-bank groups.
-    static constexpr int PJ = (NT + 3) & ~3;
-    static constexpr int PJW = PJ;
-    static constexpr int CQ = (AW_TX + 2 * R + 2 + 3) / 4; // cost quads per disparity row
-    static constexpr int PC = HALF ? (2 * CQ) | 1 : (4 * CQ) | 1;   // cost row pitch in words (odd)
-    static constexpr int C_WORDS = (64 * PC + 3) & ~3;
-    static constexpr int T_WORDS = AW_NY * 128 * PJ;
-    static constexpr int L_WORDS = AW_NY * 64 * PJW;
+    static constexpr int NTHR = 512;
+    static constexpr int TXP = 64;                          // pixels per tile
+    static constexpr int QW = TXP + 64;                     // target centres staged
+    static constexpr int NT = HALF ? R + 1 : 2 * R + 1;     // weight words per (pixel, window row)
+    static constexpr int PG = (NT + 3) & ~3;                // ... padded to 16-byte words
+    static constexpr int CQ = (TXP + 2 * R + 2 + 3) / 4;    // cost quads per disparity row (<= 32)
+    // Cost rows: FP32 floats; FP16 two copies of half pairs, one starting
+    // at an even column and one at an odd column, so that every tap pair of
+    // every pixel is one aligned word.  Odd pitch: no bank conflicts.
+    static constexpr int PC = HALF ? (2 * CQ) | 1 : (4 * CQ) | 1;
+    static constexpr int C_WORDS = ((HALF ? 2 : 1) * 64 * PC + 3) & ~3;
+    static constexpr int T_WORDS = AW_NY * QW * PG;
+    static constexpr int L_WORDS = AW_NY * TXP * PG;
     static constexpr int WORDS = (C_WORDS + T_WORDS + L_WORDS) > 64 * 65 ? C_WORDS + T_WORDS + L_WORDS : 64 * 65;
-    static constexpr int SQ_C = 4;                                   // 64 rows x 32 quad slots / 512
-    static constexpr int SQ_T = (AW_NY * NT * 32 + 511) / 512;       // target weights: 128 centres per tap word
-    static constexpr int SQ_L = (AW_NY * NT * 16 + 511) / 512;       // reference weights: 64 pixels per tap word
+    static constexpr int SQ_C = 64 * 32 / NTHR;                        // 64 rows x 32 quad slots
+    static constexpr int SQ_T = (AW_NY * QW * PG / 4 + NTHR - 1) / NTHR;
+    static constexpr int SQ_L = (AW_NY * TXP * PG / 4 + NTHR - 1) / NTHR;
 };
 
-// One window row's worth of tile inputs, held in registers between the
-// global loads and the LDS stores.
-template <bool HALF, int R>
-struct AwStage {
-    using T = AwTile<HALF, R>;
-    uint4 c[T::SQ_C], t[T::SQ_T], l[T::SQ_L];
-};
+// Tile pixel of a wave's k-th lane column.  FP16: the two waves of a
+// 16-pixel group take its even and its odd pixels, so that a lane's pixels
+// share one column parity.  FP32: 8 consecutive pixels.
+template <bool HALF>
+__device__ __forceinline__ int aw_px(int wave, int k) {
+    return HALF ? 16 * (wave >> 1) + 2 * k + (wave & 1) : 8 * wave + k;
+}
 
 template <bool HALF, int R>
 __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb,
                                                  const float *__restrict__ vol, float *__restrict__ out, int W, int H,
                                                  int D, int Wp, int y0, int rows, int r, int nxb, int nyb, int ndb) {
     using T = AwTile<HALF, R>;
-    constexpr int NT = T::NT;
+    constexpr int NT = T::NT, PG = T::PG, NTHR = T::NTHR, TXP = T::TXP, QW = T::QW;
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     uint32_t *Ct = lds, *Tt = lds + T::C_WORDS, *Lt = lds + T::C_WORDS + T::T_WORDS;
 
@@ -139,118 +164,107 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
     const int lin = xcd_order(blockIdx.x, nblk);
     const int xb = lin % nxb, rest = lin / nxb;
     const int db = rest % ndb, yb = rest / ndb;
-    const int x0 = xb * AW_TX, d0 = db * 64, ry0 = yb * AW_NY, yt = y0 + ry0;
+    const int x0 = xb * TXP, d0 = db * 64, ry0 = yb * AW_NY, yt = y0 + ry0;
     const int tid = threadIdx.x, dl = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int bx = wave * AW_NX;
     const int nrows = min(AW_NY, rows - ry0);
     const int e = 2 * r + 1;
     const size_t pl = plane(W, H);
     const int q_lo = x0 - d0 - 64;   // first target centre staged
     const bool fast_c = (W & 3) == 0 && ((x0 - R) & 3) == 0 && x0 - R >= 0 && x0 - R + 4 * T::CQ <= W;
-    const bool fast_t = q_lo >= 0 && q_lo + 128 <= W;
 
     auto row_valid = [&](int yy, int ny) {
         const int dy = yy - (yt + ny);
         return ny < nrows && dy >= -r && dy <= r;
     };
-    auto wrow = [&](const uint32_t *base, int yy, int ny, int j) {
-        return base + ((size_t)(ry0 + ny) * e + (yy - (yt + ny) + r)) * NT * Wp + (size_t)j * Wp;
+    // Weight row (pixel or centre c) of tile row ny for window row yy.
+    auto wrow = [&](const uint32_t *base, int yy, int ny, int c) {
+        return base + (((size_t)(ry0 + ny) * e + (yy - (yt + ny) + r)) * Wp + c) * PG;
     };
 
-    auto load = [&](AwStage<HALF, R> &st, int yy) {
+    // Global -> registers -> LDS for window row yy: the cost tile (columns
+    // x0-R .. of the block's 64 disparities, wrapped) and the target weight
+    // rows of the QW centres from x0-d0-64 (wrapped); FP32 also the
+    // reference weight rows of the tile's pixels.  The loads for the next
+    // row are issued before the arithmetic on this one.
+    float4 rc[T::SQ_C];
+    uint4 rt[T::SQ_T], rl[T::SQ_L];
+    auto load = [&](int yy) {
         const float *crow = vol + (size_t)wrap_any(yy, H) * W;
 #pragma unroll
         for (int s = 0; s < T::SQ_C; s++) {
-            const int dd = (wave << 3) + (s << 1) + (dl >> 5), qc = dl & 31, d = d0 + dd;
-            uint4 v = {0u, 0u, 0u, 0u};
+            const int u = s * NTHR + tid, dd = u >> 5, qc = u & 31, d = d0 + dd;
+            float4 v = {0.0f, 0.0f, 0.0f, 0.0f};
             if (qc < T::CQ && d < D) {
                 const float *src = crow + (size_t)d * pl;
                 const int c0 = x0 - R + 4 * qc;
                 if (fast_c) {
-                    v = *reinterpret_cast<const uint4 *>(src + c0);
+                    v = *reinterpret_cast<const float4 *>(src + c0);
                 } else {
-                    v.x = __float_as_uint(src[wrap_any(c0, W)]);
-                    v.y = __float_as_uint(src[wrap_any(c0 + 1, W)]);
-                    v.z = __float_as_uint(src[wrap_any(c0 + 2, W)]);
-                    v.w = __float_as_uint(src[wrap_any(c0 + 3, W)]);
+                    v.x = src[wrap_any(c0, W)];
+                    v.y = src[wrap_any(c0 + 1, W)];
+                    v.z = src[wrap_any(c0 + 2, W)];
+                    v.w = src[wrap_any(c0 + 3, W)];
                 }
             }
-            st.c[s] = v;
+            rc[s] = v;
         }
 #pragma unroll
         for (int s = 0; s < T::SQ_T; s++) {
-            const int k = s * 512 + tid;
-            const int qq = k & 31, jj = k >> 5, ny = jj >= NT ? 1 : 0, j = jj - ny * NT;
+            const int u = s * NTHR + tid, ny = u / (QW * PG / 4), v4 = u % (QW * PG / 4);
+            const int q = v4 / (PG / 4), g = v4 % (PG / 4);
             uint4 v = {0u, 0u, 0u, 0u};
-            if (k < AW_NY * NT * 32 && row_valid(yy, ny)) {
-                const uint32_t *src = wrow(wtb, yy, ny, j);
-                const int q = q_lo + 4 * qq;
-                if (fast_t) {
-                    v = *reinterpret_cast<const uint4 *>(src + q);
-                } else {
-                    v.x = src[wrap_any(q, W)];
-                    v.y = src[wrap_any(q + 1, W)];
-                    v.z = src[wrap_any(q + 2, W)];
-                    v.w = src[wrap_any(q + 3, W)];
-                }
-            }
-            st.t[s] = v;
+            if (u < AW_NY * QW * PG / 4 && row_valid(yy, ny))
+                v = reinterpret_cast<const uint4 *>(wrow(wtb, yy, ny, wrap_any(q_lo + q, W)))[g];
+            rt[s] = v;
         }
 #pragma unroll
         for (int s = 0; s < T::SQ_L; s++) {
-            const int k = s * 512 + tid;
-            const int pq = k & 15, jj = k >> 4, ny = jj >= NT ? 1 : 0, j = jj - ny * NT;
+            const int u = s * NTHR + tid, ny = u / (TXP * PG / 4), v4 = u % (TXP * PG / 4);
+            const int px = v4 / (PG / 4), g = v4 % (PG / 4);
             uint4 v = {0u, 0u, 0u, 0u};
-            if (k < AW_NY * NT * 16 && row_valid(yy, ny))
-                v = *reinterpret_cast<const uint4 *>(wrow(wlb, yy, ny, j) + x0 + 4 * pq);
-            st.l[s] = v;
+            if (u < AW_NY * TXP * PG / 4 && row_valid(yy, ny))
+                v = reinterpret_cast<const uint4 *>(wrow(wlb, yy, ny, x0 + px))[g];
+            rl[s] = v;
         }
     };
-
-    auto store = [&](const AwStage<HALF, R> &st) {
+    auto store = [&]() {
 #pragma unroll
         for (int s = 0; s < T::SQ_C; s++) {
-            const int dd = (wave << 3) + (s << 1) + (dl >> 5), qc = dl & 31;
+            const int u = s * NTHR + tid, dd = u >> 5, qc = u & 31;
             if (qc >= T::CQ) continue;
-            const uint4 v = st.c[s];
+            const float4 v = rc[s];
             if (HALF) {
+                _Float16 *even = reinterpret_cast<_Float16 *>(Ct + dd * T::PC);
+                _Float16 *odd = reinterpret_cast<_Float16 *>(Ct + (64 + dd) * T::PC) - 1;   // half i holds column i+1
                 h2 a, b;
-                a.x = (_Float16)__uint_as_float(v.x);
-                a.y = (_Float16)__uint_as_float(v.y);
-                b.x = (_Float16)__uint_as_float(v.z);
-                b.y = (_Float16)__uint_as_float(v.w);
-                Ct[dd * T::PC + 2 * qc] = __builtin_bit_cast(uint32_t, a);
-                Ct[dd * T::PC + 2 * qc + 1] = __builtin_bit_cast(uint32_t, b);
+                a.x = (_Float16)v.x;
+                a.y = (_Float16)v.y;
+                b.x = (_Float16)v.z;
+                b.y = (_Float16)v.w;
+                reinterpret_cast<h2 *>(even)[2 * qc] = a;
+                reinterpret_cast<h2 *>(even)[2 * qc + 1] = b;
+                if (qc > 0) odd[4 * qc] = a.x;
+                odd[4 * qc + 1] = a.y;
+                odd[4 * qc + 2] = b.x;
+                odd[4 * qc + 3] = b.y;
             } else {
                 uint32_t *dst = Ct + dd * T::PC + 4 * qc;
-                dst[0] = v.x;
-                dst[1] = v.y;
-                dst[2] = v.z;
-                dst[3] = v.w;
+                dst[0] = __float_as_uint(v.x);
+                dst[1] = __float_as_uint(v.y);
+                dst[2] = __float_as_uint(v.z);
+                dst[3] = __float_as_uint(v.w);
             }
         }
 #pragma unroll
         for (int s = 0; s < T::SQ_T; s++) {
-            const int k = s * 512 + tid;
-            if (k >= AW_NY * NT * 32) continue;
-            const int qq = k & 31, jj = k >> 5, ny = jj >= NT ? 1 : 0, j = jj - ny * NT;
-            uint32_t *dst = Tt + (ny * 128 + 4 * qq) * T::PJ + j;
-            dst[0] = st.t[s].x;
-            dst[T::PJ] = st.t[s].y;
-            dst[2 * T::PJ] = st.t[s].z;
-            dst[3 * T::PJ] = st.t[s].w;
+            const int u = s * NTHR + tid;
+            if (u < AW_NY * QW * PG / 4) reinterpret_cast<uint4 *>(Tt)[u] = rt[s];
         }
 #pragma unroll
         for (int s = 0; s < T::SQ_L; s++) {
-            const int k = s * 512 + tid;
-            if (k >= AW_NY * NT * 16) continue;
-            const int pq = k & 15, jj = k >> 4, ny = jj >= NT ? 1 : 0, j = jj - ny * NT;
-            uint32_t *dst = Lt + (ny * 64 + 4 * pq) * T::PJW + j;
-            dst[0] = st.l[s].x;
-            dst[T::PJW] = st.l[s].y;
-            dst[2 * T::PJW] = st.l[s].z;
-            dst[3 * T::PJW] = st.l[s].w;
+            const int u = s * NTHR + tid;
+            if (u < AW_NY * TXP * PG / 4) reinterpret_cast<uint4 *>(Lt)[u] = rl[s];
         }
     };
 
@@ -258,91 +272,105 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
 #pragma unroll
     for (int a = 0; a < AW_NY; a++)
 #pragma unroll
-        for (int m = 0; m < AW_NX; m++) num[a][m] = den[a][m] = 0.0f;
+        for (int k = 0; k < AW_NX; k++) num[a][k] = den[a][k] = 0.0f;
+
+    // FP16: per (lane column k, tile row ny) both weight rows come from LDS
+    // as 16-byte reads (the reference row is the same address for all lanes
+    // of the wave); all products are formed before the dot products, since
+    // a dot2 that reads the product written just before it costs a wait
+    // state.
+    auto compute_half = [&](int yy, const h2 *cw) {
+        const h2 one = {(_Float16)1.0f, (_Float16)1.0f};
+#pragma unroll
+        for (int ny = 0; ny < AW_NY; ny++) {
+            if (!row_valid(yy, ny)) continue;
+#pragma unroll
+            for (int k = 0; k < AW_NX; k++) {
+                const int px = aw_px<true>(wave, k);
+                const uint4 *tw =
+                    reinterpret_cast<const uint4 *>(lds + opaque(T::C_WORDS + (ny * QW + px - dl + 64) * PG));
+                const uint4 *lw = reinterpret_cast<const uint4 *>(Lt + (ny * TXP + px) * PG);
+                uint4 tv[PG / 4], lv[PG / 4];
+#pragma unroll
+                for (int g = 0; g < PG / 4; g++) {
+                    tv[g] = tw[g];
+                    lv[g] = lw[g];
+                }
+                const uint32_t *t32 = reinterpret_cast<const uint32_t *>(tv);
+                const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lv);
+                h2 p[NT];
+#pragma unroll
+                for (int j = 0; j < NT; j++) p[j] = __builtin_bit_cast(h2, l32[j]) * __builtin_bit_cast(h2, t32[j]);
+                float sn = num[ny][k], sd = den[ny][k];
+#pragma unroll
+                for (int j = 0; j < NT; j++) {
+                    sn = __builtin_amdgcn_fdot2(p[j], cw[k + j], sn, false);
+                    sd = __builtin_amdgcn_fdot2(p[j], one, sd, false);
+                }
+                num[ny][k] = sn;
+                den[ny][k] = sd;
+            }
+        }
+    };
 
     auto compute = [&](int yy) {
+        const bool v0 = row_valid(yy, 0), v1 = row_valid(yy, 1);
+        if (!v0 && !v1) return;
         if (HALF) {
-            h2 ce[AW_NX / 2 + R + 1], co[AW_NX / 2 + R];
-            const uint32_t *crow = Ct + dl * T::PC + bx / 2;
+            // Tap pair j of lane column k covers tile columns px+2j, px+2j+1;
+            // px = 16*(wave>>1) + parity + 2k, so it is word 8*(wave>>1)+k+j
+            // of the parity's copy.
+            h2 cw[AW_NX + R];
+            const uint32_t *crow = Ct + ((wave & 1) * 64 + dl) * T::PC + 8 * (wave >> 1);
 #pragma unroll
-            for (int i = 0; i < AW_NX / 2 + R + 1; i++) ce[i] = __builtin_bit_cast(h2, crow[i]);
-#pragma unroll
-            for (int i = 0; i < AW_NX / 2 + R; i++) co[i] = __builtin_shufflevector(ce[i], ce[i + 1], 1, 2);
-            const h2 one = {(_Float16)1.0f, (_Float16)1.0f};
-#pragma unroll
-            for (int ny = 0; ny < AW_NY; ny++) {
-                if (!row_valid(yy, ny)) continue;
-                const uint4 *wt = reinterpret_cast<const uint4 *>(Tt + (ny * 128 + bx - dl + 64) * T::PJ);
-                const uint4 *wl = reinterpret_cast<const uint4 *>(Lt + (ny * 64 + bx) * T::PJW);
-#pragma unroll
-                for (int m = 0; m < AW_NX; m++) {
-                    uint4 tv[T::PJ / 4], lv[T::PJ / 4];
-#pragma unroll
-                    for (int k = 0; k < T::PJ / 4; k++) {
-                        tv[k] = wt[m * (T::PJ / 4) + k];
-                        lv[k] = wl[m * (T::PJW / 4) + k];
-                    }
-                    const uint32_t *tw = reinterpret_cast<const uint32_t *>(tv);
-                    const uint32_t *lw = reinterpret_cast<const uint32_t *>(lv);
-                    // All products first: a dot2 reading a product written by
-                    // the instruction just before it costs a wait state.
-                    h2 p[NT];
-#pragma unroll
-                    for (int j = 0; j < NT; j++) p[j] = __builtin_bit_cast(h2, lw[j]) * __builtin_bit_cast(h2, tw[j]);
-                    float sn = num[ny][m], sd = den[ny][m];
-#pragma unroll
-                    for (int j = 0; j < NT; j++) {
-                        const h2 c = (m & 1) ? co[(m >> 1) + j] : ce[(m >> 1) + j];
-                        sn = __builtin_amdgcn_fdot2(p[j], c, sn, false);
-                        sd = __builtin_amdgcn_fdot2(p[j], one, sd, false);
-                    }
-                    num[ny][m] = sn;
-                    den[ny][m] = sd;
-                }
-            }
+            for (int i = 0; i < AW_NX + R; i++) cw[i] = __builtin_bit_cast(h2, crow[i]);
+            compute_half(yy, cw);
         } else {
             float cf[AW_NX + 2 * R];
-            const float *crow = reinterpret_cast<const float *>(Ct) + dl * T::PC + bx;
+            const float *crow = reinterpret_cast<const float *>(Ct) + dl * T::PC + 8 * wave;
 #pragma unroll
             for (int i = 0; i < AW_NX + 2 * R; i++) cf[i] = crow[i];
 #pragma unroll
             for (int ny = 0; ny < AW_NY; ny++) {
                 if (!row_valid(yy, ny)) continue;
-                const float4 *wt = reinterpret_cast<const float4 *>(Tt + (ny * 128 + bx - dl + 64) * T::PJ);
-                const float4 *wl = reinterpret_cast<const float4 *>(Lt + (ny * 64 + bx) * T::PJW);
 #pragma unroll
-                for (int m = 0; m < AW_NX; m++) {
-                    float4 tv[T::PJ / 4], lv[T::PJ / 4];
+                for (int k = 0; k < AW_NX; k++) {
+                    const int px = aw_px<false>(wave, k);
+                    const float4 *tw =
+                        reinterpret_cast<const float4 *>(lds + opaque(T::C_WORDS + (ny * QW + px - dl + 64) * PG));
+                    const float4 *lw = reinterpret_cast<const float4 *>(Lt + (ny * TXP + px) * PG);
+                    float4 tv[PG / 4], lv[PG / 4];
 #pragma unroll
-                    for (int k = 0; k < T::PJ / 4; k++) {
-                        tv[k] = wt[m * (T::PJ / 4) + k];
-                        lv[k] = wl[m * (T::PJW / 4) + k];
+                    for (int g = 0; g < PG / 4; g++) {
+                        tv[g] = tw[g];
+                        lv[g] = lw[g];
                     }
-                    const float *tw = reinterpret_cast<const float *>(tv);
-                    const float *lw = reinterpret_cast<const float *>(lv);
-                    float sn = num[ny][m], sd = den[ny][m];
+                    const float *tf = reinterpret_cast<const float *>(tv);
+                    const float *lf = reinterpret_cast<const float *>(lv);
+                    float sn = num[ny][k], sd = den[ny][k];
 #pragma unroll
                     for (int t = 0; t < NT; t++) {
-                        const float p = lw[t] * tw[t];
+                        const float p = lf[t] * tf[t];
                         sd += p;
-                        sn = __builtin_fmaf(p, cf[m + t], sn);
+                        sn = __builtin_fmaf(p, cf[k + t], sn);
                     }
-                    num[ny][m] = sn;
-                    den[ny][m] = sd;
+                    num[ny][k] = sn;
+                    den[ny][k] = sd;
                 }
             }
         }
     };
 
+    // FP16 prefetches the next window row into registers during the
+    // arithmetic; FP32 (more live registers) loads and stores back to back.
     const int ylast = yt + nrows - 1 + r;
-    AwStage<HALF, R> st;
-    if (HALF) load(st, yt - r);
+    if (HALF) load(yt - r);
     for (int yy = yt - r; yy <= ylast; yy++) {
         __syncthreads();
-        if (!HALF) load(st, yy);
-        store(st);
+        if (!HALF) load(yy);
+        store();
         __syncthreads();
-        if (HALF && yy < ylast) load(st, yy + 1);   // in flight during the arithmetic
+        if (HALF && yy < ylast) load(yy + 1);
         compute(yy);
     }
 
@@ -354,11 +382,11 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
         if (ny >= nrows) break;
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < AW_NX; m++) tile[dl * 65 + bx + m] = num[ny][m] / den[ny][m];
+        for (int k = 0; k < AW_NX; k++) tile[dl * 65 + aw_px<HALF>(wave, k)] = num[ny][k] / den[ny][k];
         __syncthreads();
         const int y = yt + ny;
-        for (int i = tid; i < 64 * AW_TX; i += 512) {
-            const int dd = i >> 6, px = i & 63;
+        for (int i = tid; i < 64 * TXP; i += NTHR) {
+            const int dd = i / TXP, px = i % TXP;
             const int d = d0 + dd, x = x0 + px;
             if (d < D && x < W) out[(size_t)d * pl + (size_t)y * W + x] = tile[dd * 65 + px];
         }
@@ -376,16 +404,16 @@ int aw_band_launch(const void *wl, const void *wt, const float *vol, float *out,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
         attr_set = true;
     }
-    const int nxb = div_up(W, AW_TX), nyb = div_up(rows, AW_NY), ndb = div_up(D, 64);
-    k_aw_band<HALF, R><<<nxb * nyb * ndb, 512, lds_bytes, s>>>(static_cast<const uint32_t *>(wl),
-                                                               static_cast<const uint32_t *>(wt), vol, out, W, H, D,
-                                                               Wp, y0, rows, r, nxb, nyb, ndb);
+    const int nxb = div_up(W, T::TXP), nyb = div_up(rows, AW_NY), ndb = div_up(D, 64);
+    k_aw_band<HALF, R><<<nxb * nyb * ndb, T::NTHR, lds_bytes, s>>>(static_cast<const uint32_t *>(wl),
+                                                                   static_cast<const uint32_t *>(wt), vol, out, W, H,
+                                                                   D, Wp, y0, rows, r, nxb, nyb, ndb);
     RTS_LAUNCHED();
     return RTS_OK;
 }
 
 struct AwPlan {
-    int R, NT, Wp, band, ntap_bytes;
+    int R, NT, PG, Wp, band;
     size_t view_bytes;   // workspace of one view for one band
 };
 
@@ -393,10 +421,11 @@ AwPlan aw_plan(int W, int H, int r, bool half) {
     AwPlan p;
     p.R = aw_rmax_for(r);
     p.NT = aw_ntap(p.R, half);
+    p.PG = (p.NT + 3) & ~3;
     p.Wp = div_up(W, AW_TX) * AW_TX;
-    const size_t row = (size_t)(2 * r + 1) * p.NT * p.Wp * 4;   // one view, one image row
+    const size_t row = (size_t)(2 * r + 1) * p.PG * p.Wp * 4;   // one view, one image row
     int band = (int)std::min<size_t>((size_t)H, std::max<size_t>(AW_NY, AW_WS_CAP / (2 * row)));
-    band = std::min(band, 65535 / (2 * r + 1));   // grid.y of the weight kernel
+    band = std::min(band, 4 * 65535 / (2 * r + 1));   // grid.y of the weight kernel
     band -= band % AW_NY;
     if (band < AW_NY) band = AW_NY;
     if (band > H) band = H;
@@ -411,11 +440,14 @@ int aw_run(const uint8_t *labL, const uint8_t *labR, const float *vol, float *ou
     void *wl = ws, *wt = ws + pl.view_bytes;
     for (int y0 = 0; y0 < H; y0 += pl.band) {
         const int rows = std::min(pl.band, H - y0);
-        const dim3 grid(pl.Wp / 64, rows * (2 * r + 1));
+        const int nrow = rows * (2 * r + 1);
+        const dim3 grid(pl.Wp / 64, div_up(nrow, 4));
         const float kc = (float)(-1.4426950408889634 / (double)gc);
-        k_aw_weights_band<HALF, R><<<grid, 256, 0, s>>>(labL, static_cast<uint32_t *>(wl), W, H, pl.Wp, y0, r, kc, gp);
+        k_aw_weights_band<HALF, R>
+            <<<grid, 256, 0, s>>>(labL, static_cast<uint32_t *>(wl), W, H, pl.Wp, y0, nrow, r, kc, gp);
         RTS_LAUNCHED();
-        k_aw_weights_band<HALF, R><<<grid, 256, 0, s>>>(labR, static_cast<uint32_t *>(wt), W, H, pl.Wp, y0, r, kc, gp);
+        k_aw_weights_band<HALF, R>
+            <<<grid, 256, 0, s>>>(labR, static_cast<uint32_t *>(wt), W, H, pl.Wp, y0, nrow, r, kc, gp);
         RTS_LAUNCHED();
         if (int st = aw_band_launch<HALF, R>(wl, wt, vol, out, W, H, D, pl.Wp, y0, rows, r, s)) return st;
     }
