@@ -59,6 +59,7 @@ def lib():
             "ora_compute_weights": [P, P, P, F, I, I, I],
             "ora_aw_l2r": [P, P, P, P, I, I, I, I],
             "ora_aw_l2r_f64": [P, P, P, P, P, I, P, I, I, I, I, D_, D_],
+            "ora_aw_rows_f64": [P, P, P, P, I, P, I, I, I, I, D_, D_],
             "ora_find_cross": [P, P, P, I, I, I, I],
             "ora_hprefix": [P, I, I, I],
             "ora_cross_v": [P, P, P, P, P, P, I, I, I],
@@ -214,6 +215,17 @@ def aw_l2r_f64(labL, labR, cost, ys, xs, r, gc, gp):
     out = np.zeros((len(ys), D), np.float64)
     lib().ora_aw_l2r_f64(_p(labL), _p(labR), _p(np.ascontiguousarray(cost, np.float32)), _p(ys), _p(xs), len(ys),
                          _p(out), H, W, D, r, gc, gp)
+    return out
+
+
+def aw_rows_f64(labL, labR, cost, rows, r, gc, gp):
+    """FP64 adaptive-weight aggregation of whole rows: [len(rows)][W][D]."""
+    H, W, D = cost.shape
+    assert r <= 16
+    rows = np.ascontiguousarray(rows, np.int32)
+    out = np.zeros((len(rows), W, D), np.float64)
+    lib().ora_aw_rows_f64(_p(labL), _p(labR), _p(np.ascontiguousarray(cost, np.float32)), _p(rows), len(rows),
+                          _p(out), H, W, D, r, gc, gp)
     return out
 
 

@@ -439,6 +439,68 @@ void ora_aw_l2r_f64(const uint8_t *labL, const uint8_t *labR, const float *cost,
     free(prox);
 }
 
+/* ora_aw_l2r_f64 for whole rows: out[i][x][d] for y = rows[i], every x and
+ * d, with each row's target weights computed once.  Rows run one after the
+ * other and pixels in parallel (OpenMP when built with it). */
+void ora_aw_rows_f64(const uint8_t *labL, const uint8_t *labR, const float *cost, const int *rows, int nrows,
+                     double *out, int H, int W, int D, int r, double gamma_c, double gamma_p)
+{
+    const int e = 2 * r + 1, A = e * e;
+    double *prox = (double *)malloc(sizeof(double) * A);
+    for (int dy = -r, k = 0; dy <= r; dy++)
+        for (int dx = -r; dx <= r; dx++, k++) prox[k] = exp(-sqrt((double)(dy * dy + dx * dx)) / gamma_p);
+    double *wt = (double *)malloc(sizeof(double) * (size_t)A * W);
+    for (int i = 0; i < nrows; i++) {
+        const int y = rows[i];
+#pragma omp parallel for schedule(static)
+        for (int x = 0; x < W; x++) {
+            const uint8_t *c = labR + 3 * IDX(y, x, W);
+            for (int dy = -r, k = 0; dy <= r; dy++)
+                for (int dx = -r; dx <= r; dx++, k++) {
+                    const int yy = y + dy, xx = x + dx;
+                    double v = 0.0;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                        const uint8_t *t = labR + 3 * IDX(yy, xx, W);
+                        const double e0 = (double)c[0] - t[0], e1 = (double)c[1] - t[1], e2 = (double)c[2] - t[2];
+                        v = prox[k] * exp(-sqrt(e0 * e0 + e1 * e1 + e2 * e2) / gamma_c);
+                    }
+                    wt[(size_t)x * A + k] = v;
+                }
+        }
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int x = 0; x < W; x++) {
+            double wl[33 * 33];
+            const uint8_t *c = labL + 3 * IDX(y, x, W);
+            for (int dy = -r, k = 0; dy <= r; dy++)
+                for (int dx = -r; dx <= r; dx++, k++) {
+                    const int yy = y + dy, xx = x + dx;
+                    double v = 0.0;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                        const uint8_t *t = labL + 3 * IDX(yy, xx, W);
+                        const double e0 = (double)c[0] - t[0], e1 = (double)c[1] - t[1], e2 = (double)c[2] - t[2];
+                        v = prox[k] * exp(-sqrt(e0 * e0 + e1 * e1 + e2 * e2) / gamma_c);
+                    }
+                    wl[k] = v;
+                }
+            for (int d = 0; d < D; d++) {
+                const double *wq = wt + (size_t)wrap1(x - d, W) * A;
+                double sw = 0.0, sc = 0.0;
+                for (int dy = -r, k = 0; dy <= r; dy++) {
+                    const float *crow = cost + (size_t)D * IDX(wrap1(y + dy, H), 0, W) + d;
+                    for (int dx = -r; dx <= r; dx++, k++) {
+                        const double wgt = wl[k] * wq[k];
+                        sw += wgt;
+                        sc += wgt * (double)crow[(size_t)D * wrap1(x + dx, W)];
+                    }
+                }
+                out[((size_t)i * W + x) * D + d] = sc / sw;
+            }
+        }
+    }
+    free(wt);
+    free(prox);
+}
+
 /* ------------------------------------------------------------ cross-based */
 
 /* Maximum channel |difference| between two colour pixels. */
