@@ -36,7 +36,12 @@ PEAK_HBM_GBS = 8000.0
 # the fused cost + row-prefix kernel writes the prefix volume once; the
 # vertical sweep reads it and writes the aggregated volume; WTA reads that.
 STAGE_BYTES_PER_PD = {"cost": 4.0, "aggr": 8.0, "wta": 4.0}
-STAGE_KERNEL = {"cost": "k_front_hscan", "aggr": "k_cross_v", "wta": "k_wta"}
+STAGE_KERNEL = {"cost": "k_front_hscan", "aggr": "k_cross_strip", "wta": "k_wta"}
+# HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
+# the copy kernel's reading of the same counters (FETCH x2.000, WRITE x1.000);
+# profiles/census_cross_1080p_d192.md holds the runs.
+STAGE_TRAFFIC = {"cost": (30493.2 * 2 + 1555200.0) * 1024, "aggr": (2207577.6 * 2 + 1555200.0) * 1024,
+                 "wta": (890962.5 * 2 + 6658.3) * 1024}
 
 
 def cpu_baseline(target_s: float) -> dict:
@@ -185,7 +190,7 @@ def main() -> int:
         "stage_ms": stage,
         "gt_agreement": gt_agree,
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": STAGE_TRAFFIC[dom],
                      "bytes_per_pd": STAGE_BYTES_PER_PD[dom], "algorithmic_bytes_per_launch": bytes_launch,
                      "kernel_ms": stage[dom]},
         "copy_kernel_gbs": copy,
