@@ -127,8 +127,8 @@ rts_status rts_convolve(const uint8_t *img, float *out, const float *mask_host, 
 
 /* NCC cost.  seq/NCCDisparity.cpp:11-73 calcNCCDispL2R.
  * RTS_NCC_SEQ: the reference's float window sums in its order (bit-exact).
- * RTS_NCC_INTEGRAL: exact integer window sums (sliding sums, int64) combined
- *   in FP64, written as float.
+ * RTS_NCC_INTEGRAL: exact integer window sums (int64, the denominator in
+ *   128-bit integers) combined in FP64, written as float.
  * RTS_NCC_KEEP_FLAT: where the denominator is 0 leave vol untouched (the
  *   reference); otherwise write 0. */
 enum { RTS_NCC_SEQ = 0, RTS_NCC_INTEGRAL = 1, RTS_NCC_KEEP_FLAT = 0x100 };
@@ -138,6 +138,24 @@ rts_status rts_ncc(const uint8_t *ref, const uint8_t *tar, float *vol, int W, in
 /* In place c1 <- (1-exp(-c1/l1)) + (1-exp(-c2/l2)) over n elements.
  * seq/CombineCost.cpp:11-25 combineCost. */
 rts_status rts_combine(float *c1, const float *c2, float l1, float l2, size_t n, void *stream);
+
+/* The three-term cost of rts_pipeline_desc.cost3 in one pass (no reference
+ * counterpart; built from seq/RankFilter.cpp + PixelBasedCost.cpp:13-24,
+ * ConvolveImage.cpp + PixelBasedCost.cpp:39-53, NCCDisparity.cpp:11-73 and
+ * the CombineCost.cpp:11-25 form):
+ *   vol = (1-exp(-|rank_ref-rank_tar|/l1)) + (1-exp(-|log_ref-log_tar|/l2))
+ *       + (1-exp(-(1-ncc)/l3)),   tar at x-d (wrapped),
+ * ncc over a (2hr+1)x(2wr+1) window of the gray images from exact integer
+ * sums (0 for a flat window).  hr, wr <= 5. */
+rts_status rts_cost_rank_log_ncc(const uint8_t *gray_ref, const uint8_t *gray_tar, const uint8_t *rank_ref,
+                                 const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol, int W,
+                                 int H, int D, int hr, int wr, float l1, float l2, float l3, void *stream);
+/* The same with the disparities per block (8, 16 or 32) chosen by the caller
+ * (the batch autotuner); rts_cost_rank_log_ncc uses 16. */
+rts_status rts_cost_rank_log_ncc_dch(const uint8_t *gray_ref, const uint8_t *gray_tar, const uint8_t *rank_ref,
+                                     const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol,
+                                     int W, int H, int D, int hr, int wr, float l1, float l2, float l3, int dch,
+                                     void *stream);
 
 /* ------------------------------------------------------------ aggregation */
 
@@ -267,6 +285,14 @@ typedef struct rts_pipeline_desc {
     int wta_literal;                /* reproduce the reference's WTA carry */
     int lr_check;                   /* compute D_R and the mask */
     int lr_tol;
+    /* Third cost term (no counterpart in the reference, which combines at
+     * most two costs, seq/CombineCost.cpp:11-25).  Only the combination
+     * cost1 = RTS_COST_RANK, cost2 = RTS_COST_LOG, cost3 = RTS_COST_NCC is
+     * accepted; the cost is then
+     *   (1-exp(-rankAD/lambda1)) + (1-exp(-LoGAD/lambda2)) + (1-exp(-(1-NCC)/lambda3))
+     * with NCC = 0 where its window is flat, and it is built in one pass. */
+    int cost3;
+    float lambda3;
 } rts_pipeline_desc;
 
 typedef struct rts_pipeline rts_pipeline;
@@ -293,6 +319,41 @@ rts_status rts_pipeline_stage_ms(rts_pipeline *p, float *ms, const char **names,
 /* Device pointer of the final aggregated float volume of the last run
  * (disparity-major), for inspection. */
 const float *rts_pipeline_volume(rts_pipeline *p);
+
+/* ------------------------------------------------------------ batches */
+
+/* Many pairs of different sizes under one configuration: the rank + LoG +
+ * NCC cost (cost3) with box aggregation (box_r <= 8), WTA, right view and
+ * check.  Pairs are cut into chunks whose device footprint
+ * (rts_batch_pair_bytes each) fits budget_bytes; in a chunk the box
+ * aggregation of all pairs is one persistent launch over a work queue, and
+ * with use_graphs the chunk's launch sequence is captured into a HIP graph
+ * on first use and replayed when the same pairs (sizes and buffers) run
+ * again (a non-null stream is then required).  The first run times the
+ * candidate tile shapes (cost disparities per block 8/16/32, box strip rows
+ * 16/32/64) on its largest pair and keeps the fastest.  All buffers are
+ * device memory; results are complete when the stream reaches the end of
+ * the run.  No reference counterpart (its driver runs one pair per
+ * process, seq/Main.cpp:47-299). */
+typedef struct rts_batch_item {
+    int width, height, disp;
+    const uint8_t *left_rgb, *right_rgb;   /* [H][W][3] */
+    uint8_t *dl, *dr, *mask;                /* [H][W] */
+} rts_batch_item;
+typedef struct rts_batch_stats {
+    int pairs, chunks;
+    int graphs_built, graph_replays;        /* cumulative */
+    unsigned long long box_items;           /* cumulative box work items */
+    size_t arena_bytes;
+    int box_rows, cost_disparities;         /* tuned tile shape */
+    float tune_ms[6];                       /* cost dch 8/16/32, box rows 16/32/64 */
+} rts_batch_stats;
+typedef struct rts_batch rts_batch;
+rts_status rts_batch_create(const rts_pipeline_desc *base, size_t budget_bytes, int use_graphs, rts_batch **out);
+rts_status rts_batch_destroy(rts_batch *b);
+rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void *stream);
+rts_status rts_batch_get_stats(rts_batch *b, rts_batch_stats *st);
+size_t rts_batch_pair_bytes(int W, int H, int D);
 
 /* ------------------------------------------------------------ serving */
 

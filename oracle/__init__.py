@@ -53,6 +53,7 @@ def lib():
             "ora_ncc": [P, P, P, I, I, I, I, I, I],
             "ora_ncc_f64": [P, P, P, I, I, I, I, I],
             "ora_combine": [P, P, F, F, I, I, I],
+            "ora_combine3": [P, P, P, F, F, F, C.c_size_t],
             "ora_combine_f64": [P, P, D_, D_, P, C.c_size_t],
             "ora_box": [P, P, I, I, I, I, I],
             "ora_prox_weight": [P, F, I],
@@ -170,6 +171,13 @@ def combine(c1, c2, l1, l2):
     H, W, D = c1.shape
     out = np.ascontiguousarray(c1, np.float32).copy()
     lib().ora_combine(_p(out), _p(np.ascontiguousarray(c2, np.float32)), l1, l2, H, W, D)
+    return out
+
+
+def combine3(c1, c2, ncc, l1, l2, l3):
+    out = np.ascontiguousarray(c1, np.float32).copy()
+    lib().ora_combine3(_p(out), _p(np.ascontiguousarray(c2, np.float32)), _p(np.ascontiguousarray(ncc, np.float32)),
+                       l1, l2, l3, out.size)
     return out
 
 
@@ -322,7 +330,9 @@ def pipeline(desc, left_rgb, right_rgb, left_gray=None, right_gray=None, left_la
         raise ValueError(which)
 
     vol = cost(d.cost1)
-    if d.cost2:
+    if getattr(d, "cost3", 0):
+        vol = combine3(vol, cost(d.cost2), cost(d.cost3), d.lambda1, d.lambda2, d.lambda3)
+    elif d.cost2:
         vol = combine(vol, cost(d.cost2), d.lambda1, d.lambda2)
     if d.aggr == 1:
         vol = box(vol, d.box_r)

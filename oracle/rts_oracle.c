@@ -290,6 +290,19 @@ void ora_combine(float *c1, const float *c2, float l1, float l2, int H, int W, i
     }
 }
 
+/* Three-term form of the batched workload (the seq/CombineCost.cpp:11-25
+ * term for each cost; the NCC similarity enters as 1 - ncc):
+ *   c1 = ((1-expf(-c1/l1)) + (1-expf(-c2/l2))) + (1-expf(-(1-ncc)/l3)). */
+void ora_combine3(float *c1, const float *c2, const float *ncc, float l1, float l2, float l3, size_t n)
+{
+    for (size_t i = 0; i < n; i++) {
+        const float a = 1 - expf(-c1[i] / l1);
+        const float b = 1 - expf(-c2[i] / l2);
+        const float c = 1 - expf(-(1 - ncc[i]) / l3);
+        c1[i] = (a + b) + c;
+    }
+}
+
 void ora_combine_f64(const float *c1, const float *c2, double l1, double l2, double *out, size_t n)
 {
     for (size_t i = 0; i < n; i++)

@@ -44,7 +44,8 @@ class PipelineDesc(C.Structure):
                 ("cross_L", i32), ("cross_tao", i32), ("cross_iters", i32), ("cross_exact", i32),
                 ("aw_r", i32), ("aw_gamma_c", f32), ("aw_gamma_p", f32), ("aw_precision", i32),
                 ("wta_literal", i32),
-                ("lr_check", i32), ("lr_tol", i32)]
+                ("lr_check", i32), ("lr_tol", i32),
+                ("cost3", i32), ("lambda3", f32)]
 
 
 # name -> argtypes (all return rts_status unless listed in _RESTYPES)
@@ -89,6 +90,8 @@ _SIGS = {
     "rts_pipeline_defaults": [C.POINTER(PipelineDesc), i32, i32, i32],
     "rts_pipeline_bytes": [C.POINTER(PipelineDesc)],
     "rts_aw_workspace_bytes": [i32, i32, i32, i32],
+    "rts_cost_rank_log_ncc": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, vp],
+    "rts_cost_rank_log_ncc_dch": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, i32, vp],
     "rts_pipeline_create": [C.POINTER(PipelineDesc), C.POINTER(vp)],
     "rts_pipeline_destroy": [vp],
     "rts_pipeline_run": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],

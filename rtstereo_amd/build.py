@@ -18,7 +18,7 @@ OBJ = os.path.join(ROOT, "_build")
 ARCH = os.environ.get("RTS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["rts_runtime.hip", "k_cost.hip", "k_aggr.hip", "k_aw.hip", "k_wta.hip", "pipeline.hip",
-               "server.hip"]
+               "server.hip", "batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
              # packed FP32 instructions are kept out of every kernel

@@ -1,0 +1,450 @@
+// Batched execution of stereo pairs of different sizes under one pipeline
+// configuration (rank 7x7 -> AD, LoG -> AD and NCC combined in one pass,
+// box aggregation, WTA + right view + left-right check).
+//
+// The pairs are cut into chunks whose device footprint fits the byte budget.
+// Within a chunk the stages run stage by stage over all of its pairs:
+//   per pair   gray, rank and LoG images, then the fused three-term cost
+//              (disparities per block chosen by the tuner);
+//   one launch box aggregation of every pair's volume by a persistent kernel
+//              that pulls (pair, disparity, row strip, column tile) work
+//              items from a device counter (strip height chosen by the
+//              tuner), so pairs of all sizes share one grid;
+//   per pair   WTA with the right view and the check.
+// The chunk's whole launch sequence is captured into a HIP graph the first
+// time the same list of pairs (sizes and buffers) is run and replayed after.
+//
+// The reference has no counterpart: its driver runs one pair per process
+// (seq/Main.cpp:47-299).
+#include "rts_internal.h"
+
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <vector>
+
+namespace {
+
+using namespace rts;
+
+constexpr int BQ_RMAX = 8;   // box radius limit of the persistent kernel
+
+struct BoxPair {
+    const float *in;
+    float *out;
+    int W, H, D, nx, ny;
+    unsigned tile0;   // first work item of this pair
+};
+
+// Work item t of pair p: d = t / (ny*nx), strip = (t / nx) % ny, tile = t % nx.
+template <int TY>
+__global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ pairs, int npairs, unsigned ntiles,
+                                                   unsigned *__restrict__ counter, int r) {
+    __shared__ float tile[TY + 2 * BQ_RMAX][64 + 2 * BQ_RMAX + 1];
+    __shared__ float hs[TY + 2 * BQ_RMAX][64];
+    __shared__ unsigned item;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int e = 2 * r + 1;
+    const float area = (float)(e * e);
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const unsigned t = item;
+        __syncthreads();   // every thread has read the item before thread 0 takes the next
+        if (t >= ntiles) break;
+        int lo = 0, hi = npairs - 1;   // last pair whose first item is <= t
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pairs[mid].tile0 <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BoxPair bp = pairs[lo];
+        const unsigned u = t - bp.tile0;
+        const int xb = (int)(u % bp.nx), sb = (int)((u / bp.nx) % bp.ny), d = (int)(u / ((unsigned)bp.nx * bp.ny));
+        const int W = bp.W, H = bp.H, x0 = xb * 64, y0 = sb * TY;
+        const size_t base = (size_t)d * plane(W, H);
+        const int th = TY + 2 * r, tw = 64 + 2 * r;
+        for (int i = threadIdx.x; i < th * tw; i += 256) {
+            const int rr = i / tw, cc = i - rr * tw;
+            tile[rr][cc] = bp.in[base + (size_t)wrapn(y0 - r + rr, H) * W + wrapn(x0 - r + cc, W)];
+        }
+        __syncthreads();
+        for (int rr = ty; rr < th; rr += 4) {
+            float h = 0.0f;
+            for (int dx = 0; dx < e; dx++) h += tile[rr][tx + dx];
+            hs[rr][tx] = h;
+        }
+        __syncthreads();
+        const int x = x0 + tx;
+        if (x < W)
+            for (int ly = ty; ly < TY; ly += 4) {
+                const int y = y0 + ly;
+                if (y >= H) break;
+                float acc = 0.0f;
+                for (int dy = 0; dy < e; dy++) acc += hs[ly + dy][tx];
+                bp.out[base + (size_t)y * W + x] = __fdiv_rn(acc, area);
+            }
+    }
+}
+
+__global__ void k_zero_u32(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+struct PairBufs {
+    uint8_t *gray[2], *rank[2];
+    float *logimg[2], *vol, *agg;
+};
+
+struct Chunk {
+    std::vector<int> items;              // indices into the run's item list
+    std::vector<rts_batch_item> key;     // what the graph was captured for
+    std::vector<PairBufs> bufs;          // carved from the batch's arena
+    BoxPair *dpairs = nullptr;           // device table for the persistent box
+    size_t dpairs_cap = 0;
+    unsigned *counter = nullptr;
+    unsigned ntiles = 0;
+    int box_ty = 0;
+    hipGraphExec_t exec = nullptr;
+};
+
+size_t pair_bytes(const rts_batch_item &it) {
+    const size_t P = (size_t)it.width * it.height;
+    const size_t V = P * it.disp * sizeof(float);
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    return 2 * al(V) + 2 * (al(P) + al(P) + al(P * 4));
+}
+
+bool same_items(const std::vector<rts_batch_item> &a, const rts_batch_item *b, const std::vector<int> &idx) {
+    if (a.size() != idx.size()) return false;
+    for (size_t i = 0; i < idx.size(); i++)
+        if (memcmp(&a[i], &b[idx[i]], sizeof(rts_batch_item)) != 0) return false;
+    return true;
+}
+
+}  // namespace
+
+struct rts_batch {
+    rts_pipeline_desc base{};
+    size_t budget = 0;
+    // One arena, sized for the largest chunk, serves every chunk in turn:
+    // chunks run one after another on the stream, so their graphs may all
+    // point into it.
+    uint8_t *arena = nullptr;
+    size_t arena_bytes = 0;
+    int use_graphs = 1;
+    int tuned = 0, box_ty = 32, cost_dch = 16;
+    float tune_ms[6] = {};
+    std::vector<std::unique_ptr<Chunk>> chunks;
+    rts_batch_stats stats{};
+};
+
+namespace {
+
+void free_chunk(Chunk *c) {
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    (void)hipFree(c->dpairs);
+    (void)hipFree(c->counter);
+    c->exec = nullptr;
+    c->dpairs = nullptr;
+    c->dpairs_cap = 0;
+    c->counter = nullptr;
+}
+
+int launch_box(const Chunk &c, int ty, int r, hipStream_t s) {
+    const int nb = std::max(1, std::min<int>((int)std::min<unsigned>(c.ntiles, 1u << 30), 256 * 8));
+    k_zero_u32<<<1, 64, 0, s>>>(c.counter, 1);
+    RTS_LAUNCHED();
+    switch (ty) {
+        case 16: k_box_queue<16><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter, r); break;
+        case 64: k_box_queue<64><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter, r); break;
+        default: k_box_queue<32><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter, r); break;
+    }
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+// Sets the chunk's box work table for strip height ty (device upload).
+int build_box_table(Chunk &c, const rts_batch_item *items, int ty, hipStream_t s) {
+    std::vector<BoxPair> t(c.items.size());
+    unsigned n = 0;
+    for (size_t i = 0; i < c.items.size(); i++) {
+        const rts_batch_item &it = items[c.items[i]];
+        BoxPair &b = t[i];
+        b.in = c.bufs[i].vol;
+        b.out = c.bufs[i].agg;
+        b.W = it.width;
+        b.H = it.height;
+        b.D = it.disp;
+        b.nx = div_up(it.width, 64);
+        b.ny = div_up(it.height, ty);
+        b.tile0 = n;
+        const unsigned long add = (unsigned long)b.nx * b.ny * b.D;
+        RTS_CHECK(n + add < (1ul << 31), "too many box work items in one chunk");
+        n += (unsigned)add;
+    }
+    if (c.dpairs_cap < t.size()) {
+        (void)hipFree(c.dpairs);
+        c.dpairs = nullptr;
+        c.dpairs_cap = 0;
+        RTS_HIP(hipMalloc((void **)&c.dpairs, sizeof(BoxPair) * t.size()));
+        c.dpairs_cap = t.size();
+    }
+    if (!c.counter) RTS_HIP(hipMalloc((void **)&c.counter, 256));
+    RTS_HIP(hipMemcpyAsync(c.dpairs, t.data(), sizeof(BoxPair) * t.size(), hipMemcpyHostToDevice, s));
+    RTS_HIP(hipStreamSynchronize(s));
+    c.ntiles = n;
+    c.box_ty = ty;
+    return RTS_OK;
+}
+
+int cost_pair(const rts_batch *b, const rts_batch_item &it, const PairBufs &pb, int dch, hipStream_t s) {
+    const rts_pipeline_desc &d = b->base;
+    return rts_cost_rank_log_ncc_dch(pb.gray[0], pb.gray[1], pb.rank[0], pb.rank[1], pb.logimg[0], pb.logimg[1], pb.vol,
+                                     it.width, it.height, it.disp, d.ncc_hr, d.ncc_wr, d.lambda1, d.lambda2, d.lambda3,
+                                     dch, s);
+}
+
+int prep_pair(const rts_batch *b, const rts_batch_item &it, const PairBufs &pb, hipStream_t s) {
+    static const float kLoG5[25] = {0.0239f, 0.046f,   0.0499f, 0.046f,   0.0239f,  0.046f,  0.0061f, -0.0923f, 0.0061f,
+                                    0.046f,  0.0499f,  -0.0923f, -0.3182f, -0.0923f, 0.0499f, 0.046f,  0.0061f, -0.0923f,
+                                    0.0061f, 0.046f,   0.0239f, 0.046f,   0.0499f,  0.046f,  0.0239f};
+    const uint8_t *rgb[2] = {it.left_rgb, it.right_rgb};
+    for (int v = 0; v < 2; v++) {
+        if (int e = rts_rgb_to_gray(rgb[v], pb.gray[v], it.width, it.height, s)) return e;
+        if (int e = rts_rank(pb.gray[v], pb.rank[v], it.width, it.height, b->base.rank_m, s)) return e;
+        if (int e = rts_convolve(pb.gray[v], pb.logimg[v], kLoG5, it.width, it.height, 2, 2, s)) return e;
+    }
+    return RTS_OK;
+}
+
+// The chunk's whole launch sequence on stream s.
+int enqueue_chunk(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s) {
+    for (size_t i = 0; i < c.items.size(); i++) {
+        const rts_batch_item &it = items[c.items[i]];
+        if (int e = prep_pair(b, it, c.bufs[i], s)) return e;
+        if (int e = cost_pair(b, it, c.bufs[i], b->cost_dch, s)) return e;
+    }
+    if (int e = launch_box(c, c.box_ty, b->base.box_r, s)) return e;
+    for (size_t i = 0; i < c.items.size(); i++) {
+        const rts_batch_item &it = items[c.items[i]];
+        if (int e = rts_wta(c.bufs[i].agg, it.dl, it.dr, it.mask, it.width, it.height, it.disp, 0, b->base.lr_tol, s))
+            return e;
+    }
+    return RTS_OK;
+}
+
+// Carves the chunk's pair buffers out of the arena (c.items set).
+void carve(rts_batch *b, Chunk &c, const rts_batch_item *items) {
+    auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
+    uint8_t *p = b->arena;
+    c.bufs.resize(c.items.size());
+    for (size_t i = 0; i < c.items.size(); i++) {
+        const rts_batch_item &it = items[c.items[i]];
+        const size_t P = (size_t)it.width * it.height, V = P * it.disp * sizeof(float);
+        PairBufs &b = c.bufs[i];
+        b.vol = (float *)p;
+        p += al(V);
+        b.agg = (float *)p;
+        p += al(V);
+        for (int v = 0; v < 2; v++) {
+            b.gray[v] = p;
+            p += al(P);
+            b.rank[v] = p;
+            p += al(P);
+            b.logimg[v] = (float *)p;
+            p += al(P * 4);
+        }
+    }
+}
+
+// Times the candidates of each tuned parameter on the largest pair of the
+// first chunk and keeps the fastest: the box strip height (16, 32, 64 rows)
+// and the cost kernel's disparities per block (8, 16, 32).
+int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s) {
+    size_t big = 0;
+    for (size_t i = 1; i < c.items.size(); i++)
+        if (pair_bytes(items[c.items[i]]) > pair_bytes(items[c.items[big]])) big = i;
+    Chunk one;
+    one.items = {c.items[big]};
+    one.bufs = {c.bufs[big]};
+    hipEvent_t e0, e1;
+    RTS_HIP(hipEventCreate(&e0));
+    RTS_HIP(hipEventCreate(&e1));
+    int st = prep_pair(b, items[one.items[0]], one.bufs[0], s);
+    const int dchs[3] = {8, 16, 32}, tys[3] = {16, 32, 64};
+    float best = 1e30f;
+    for (int k = 0; k < 3 && st == RTS_OK; k++) {
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && st == RTS_OK; rep++) {   // first run warms up
+            st = hipEventRecord(e0, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
+            if (st == RTS_OK) st = cost_pair(b, items[one.items[0]], one.bufs[0], dchs[k], s);
+            if (st == RTS_OK) st = hipEventRecord(e1, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
+            if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        b->tune_ms[k] = ms;
+        if (ms < best) {
+            best = ms;
+            b->cost_dch = dchs[k];
+        }
+    }
+    best = 1e30f;
+    for (int k = 0; k < 3 && st == RTS_OK; k++) {
+        st = build_box_table(one, items, tys[k], s);
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && st == RTS_OK; rep++) {
+            st = hipEventRecord(e0, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
+            if (st == RTS_OK) st = launch_box(one, tys[k], b->base.box_r, s);
+            if (st == RTS_OK) st = hipEventRecord(e1, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
+            if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        b->tune_ms[3 + k] = ms;
+        if (ms < best) {
+            best = ms;
+            b->box_ty = tys[k];
+        }
+    }
+    (void)hipFree(one.dpairs);
+    (void)hipFree(one.counter);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->tuned = st == RTS_OK;
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+rts_status rts_batch_create(const rts_pipeline_desc *base, size_t budget_bytes, int use_graphs, rts_batch **out) {
+    RTS_CHECK(base && out, "null argument");
+    RTS_CHECK(base->cost1 == RTS_COST_RANK && base->cost2 == RTS_COST_LOG && base->cost3 == RTS_COST_NCC &&
+                  base->aggr == RTS_AGGR_BOX,
+              "the batch engine runs the rank + LoG + NCC cost with box aggregation");
+    RTS_CHECK(base->box_r >= 0 && base->box_r <= BQ_RMAX, "box radius %d must be in [0, %d]", base->box_r, BQ_RMAX);
+    RTS_CHECK(base->lambda1 > 0 && base->lambda2 > 0 && base->lambda3 > 0, "combine weights must be positive");
+    RTS_CHECK(budget_bytes > 0, "empty budget");
+    rts_batch *b = new (std::nothrow) rts_batch();
+    RTS_CHECK(b != nullptr, "out of host memory");
+    b->base = *base;
+    b->budget = budget_bytes;
+    b->use_graphs = use_graphs != 0;
+    *out = b;
+    return RTS_OK;
+}
+
+rts_status rts_batch_destroy(rts_batch *b) {
+    if (!b) return RTS_OK;
+    for (auto &c : b->chunks) free_chunk(c.get());
+    (void)hipFree(b->arena);
+    delete b;
+    return RTS_OK;
+}
+
+rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void *stream) {
+    RTS_CHECK(b && (items || n == 0), "null argument");
+    RTS_CHECK(n >= 0, "negative pair count");
+    hipStream_t s = S(stream);
+    for (int i = 0; i < n; i++) {
+        const rts_batch_item &it = items[i];
+        if (int e = check_dims(it.width, it.height, it.disp)) return e;
+        RTS_CHECK(it.left_rgb && it.right_rgb && it.dl && it.dr && it.mask, "pair %d: null buffer", i);
+        RTS_CHECK(2 * b->base.ncc_hr + 1 <= it.height && 2 * b->base.ncc_wr + 1 <= it.width &&
+                      b->base.box_r < it.width && b->base.box_r < it.height,
+                  "pair %d: windows larger than the image", i);
+        RTS_CHECK(pair_bytes(it) <= b->budget, "pair %d needs %zu bytes, over the budget of %zu", i, pair_bytes(it),
+                  b->budget);
+    }
+    // Chunks: consecutive pairs while their footprint fits the budget.
+    std::vector<std::vector<int>> plan;
+    size_t used = 0, arena_need = 0;
+    for (int i = 0; i < n; i++) {
+        const size_t pb = pair_bytes(items[i]);
+        if (plan.empty() || used + pb > b->budget) {
+            plan.emplace_back();
+            used = 0;
+        }
+        plan.back().push_back(i);
+        used += pb;
+        arena_need = std::max(arena_need, used);
+    }
+    RTS_CHECK(!b->use_graphs || stream != nullptr, "graph capture needs a stream other than the null stream");
+    if (b->arena_bytes < arena_need) {   // a new arena invalidates every captured graph
+        for (auto &c : b->chunks) free_chunk(c.get());
+        b->chunks.clear();
+        RTS_HIP(hipStreamSynchronize(s));
+        (void)hipFree(b->arena);
+        b->arena = nullptr;
+        b->arena_bytes = 0;
+        if (hipMalloc((void **)&b->arena, arena_need) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(RTS_ERR_NOMEM, "batch: arena of %zu bytes", arena_need);
+        }
+        b->arena_bytes = arena_need;
+    }
+    while (b->chunks.size() < plan.size()) b->chunks.emplace_back(new Chunk());
+    b->stats.pairs = n;
+    b->stats.chunks = (int)plan.size();
+    b->stats.arena_bytes = b->arena_bytes;
+    for (size_t k = 0; k < plan.size(); k++) {
+        Chunk &c = *b->chunks[k];
+        const bool reuse = c.exec && c.box_ty == b->box_ty && same_items(c.key, items, plan[k]);
+        if (!reuse) {
+            if (c.exec) (void)hipGraphExecDestroy(c.exec);
+            c.exec = nullptr;
+            c.items = plan[k];
+            c.key.clear();
+            for (int i : c.items) c.key.push_back(items[i]);
+            carve(b, c, items);
+            if (!b->tuned)
+                if (int e = autotune(b, c, items, s)) return e;
+            if (int e = build_box_table(c, items, b->box_ty, s)) return e;
+            if (b->use_graphs) {
+                hipGraph_t g = nullptr;
+                RTS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                const int e = enqueue_chunk(b, c, items, s);
+                const hipError_t ce = hipStreamEndCapture(s, &g);
+                if (e != RTS_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return e;
+                }
+                RTS_HIP(ce);
+                const hipError_t ie = hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                RTS_HIP(ie);
+                b->stats.graphs_built++;
+            }
+        } else {
+            b->stats.graph_replays++;
+        }
+        if (b->use_graphs) {
+            RTS_HIP(hipGraphLaunch(c.exec, s));
+        } else {
+            if (int e = enqueue_chunk(b, c, items, s)) return e;
+        }
+        b->stats.box_items += c.ntiles;
+    }
+    b->stats.box_rows = b->box_ty;
+    b->stats.cost_disparities = b->cost_dch;
+    for (int i = 0; i < 6; i++) b->stats.tune_ms[i] = b->tune_ms[i];
+    return RTS_OK;
+}
+
+rts_status rts_batch_get_stats(rts_batch *b, rts_batch_stats *st) {
+    RTS_CHECK(b && st, "null argument");
+    *st = b->stats;
+    return RTS_OK;
+}
+
+size_t rts_batch_pair_bytes(int W, int H, int D) {
+    rts_batch_item it{};
+    it.width = W;
+    it.height = H;
+    it.disp = D;
+    return pair_bytes(it);
+}
+
+}  // extern "C"

@@ -266,6 +266,87 @@ __global__ void __launch_bounds__(256) k_ncc_int(const uint8_t *__restrict__ ref
     }
 }
 
+// Three-term cost of the batched workload, in one pass: per (x, y, d)
+//   c = (1-exp(-|rankL(x)-rankR(x-d)|/l1)) + (1-exp(-|logL(x)-logR(x-d)|/l2))
+//     + (1-exp(-(1-ncc)/l3)),
+// ncc from the same exact integer window sums as k_ncc_int (0 where the
+// window is flat).  Block: 64 x 4 pixels x NC_DCH disparities; the gray
+// rows the block's windows touch are staged in LDS.
+constexpr int NC_RMAX = 5;
+constexpr int NC_TH = 4 + 2 * NC_RMAX;   // staged rows
+
+__device__ __forceinline__ float ncc_exact(long N, long Sa, long Saa, long Sb, long Sbb, long Sab, long Saab, long Sabb,
+                                           unsigned long Saabb) {
+    const __int128 n = N;
+    const __int128 X = n * n * n * n * (__int128)Saabb - 2 * n * n * n * ((__int128)Sb * Saab + (__int128)Sa * Sabb) +
+                       n * n * ((__int128)Sb * Sb * Saa + (__int128)Sa * Sa * Sbb + 4 * (__int128)Sa * Sb * Sab) -
+                       3 * n * (__int128)Sa * Sa * Sb * Sb;
+    const long nomN = N * Sab - Sa * Sb;
+    return X > 0 ? (float)((double)N * (double)nomN / sqrt((double)X)) : 0.0f;
+}
+
+template <int NC_DCH>
+__global__ void __launch_bounds__(256) k_cost_rank_log_ncc(const uint8_t *__restrict__ gl, const uint8_t *__restrict__ gr,
+                                                           const uint8_t *__restrict__ rkl,
+                                                           const uint8_t *__restrict__ rkr,
+                                                           const float *__restrict__ lgl, const float *__restrict__ lgr,
+                                                           float *__restrict__ vol, int W, int H, int D, int hr,
+                                                           int wr, float l1, float l2, float l3) {
+    constexpr int NC_TW = 64 + 2 * NC_RMAX + NC_DCH;   // staged columns (tar)
+    __shared__ uint8_t ta[NC_TH][64 + 2 * NC_RMAX], tb[NC_TH][NC_TW];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4, d0 = blockIdx.z * NC_DCH;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int rows = 4 + 2 * hr, acols = 64 + 2 * wr, bcols = 64 + 2 * wr + NC_DCH - 1;
+    for (int i = tid; i < rows * acols; i += 256) {
+        const int r = i / acols, c = i - r * acols;
+        ta[r][c] = gl[(size_t)wrapn(y0 - hr + r, H) * W + wrapn(x0 - wr + c, W)];
+    }
+    for (int i = tid; i < rows * bcols; i += 256) {
+        const int r = i / bcols, c = i - r * bcols;
+        tb[r][c] = gr[(size_t)wrapn(y0 - hr + r, H) * W + wrapn(x0 - wr - d0 - (NC_DCH - 1) + c, W)];
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const size_t pl = plane(W, H);
+    const long N = (long)(2 * hr + 1) * (2 * wr + 1);
+    const int ty = threadIdx.y, tx = threadIdx.x;
+    int Sa = 0, Saa = 0;
+    for (int dy = 0; dy <= 2 * hr; dy++)
+        for (int dx = 0; dx <= 2 * wr; dx++) {
+            const int a = ta[ty + dy][tx + dx];
+            Sa += a;
+            Saa += a * a;
+        }
+    const float rl = rkl[(size_t)y * W + x], ll = lgl[(size_t)y * W + x];
+    const int dend = min(D, d0 + NC_DCH);
+    for (int d = d0; d < dend; d++) {
+        // tar column of window tap dx for disparity d: x + dx - wr - d
+        const int bc = tx + (NC_DCH - 1) - (d - d0);
+        int Sb = 0, Sbb = 0, Sab = 0, Saab = 0, Sabb = 0;
+        unsigned long Saabb = 0;
+        for (int dy = 0; dy <= 2 * hr; dy++)
+            for (int dx = 0; dx <= 2 * wr; dx++) {
+                const int a = ta[ty + dy][tx + dx], b = tb[ty + dy][bc + dx];
+                const int ab = a * b;
+                Sb += b;
+                Sbb += b * b;
+                Sab += ab;
+                Saab += ab * a;
+                Sabb += ab * b;
+                Saabb += (unsigned long)((unsigned)ab * (unsigned)ab);
+            }
+        const float s = ncc_exact(N, Sa, Saa, Sb, Sbb, Sab, Saab, Sabb, Saabb);
+        const int xt = wrap1(x - d, W);
+        const float cr = fabsf(rl - (float)rkr[(size_t)y * W + xt]);
+        const float cl = fabsf(ll - lgr[(size_t)y * W + xt]);
+        const float t1 = 1.0f - expf(__fdiv_rn(-cr, l1));
+        const float t2 = 1.0f - expf(__fdiv_rn(-cl, l2));
+        const float t3 = 1.0f - expf(__fdiv_rn(-(1.0f - s), l3));
+        vol[d * pl + (size_t)y * W + x] = (t1 + t2) + t3;
+    }
+}
+
 // seq/CombineCost.cpp:11-25.
 __global__ void k_combine(float *__restrict__ c1, const float *__restrict__ c2, float l1, float l2, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -377,6 +458,38 @@ rts_status rts_ncc(const uint8_t *ref, const uint8_t *tar, float *vol, int W, in
         k_ncc_int<<<grid, block, 0, S(stream)>>>(ref, tar, vol, W, H, D, hr, wr, keep);
     RTS_LAUNCHED();
     return RTS_OK;
+}
+rts_status rts_cost_rank_log_ncc_dch(const uint8_t *gray_ref, const uint8_t *gray_tar, const uint8_t *rank_ref,
+                                     const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol,
+                                     int W, int H, int D, int hr, int wr, float l1, float l2, float l3, int dch,
+                                     void *stream) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(hr >= 0 && wr >= 0 && hr <= NC_RMAX && wr <= NC_RMAX, "NCC radii (%d, %d) must be in [0, %d]", hr, wr,
+              NC_RMAX);
+    RTS_CHECK(2 * hr + 1 <= H && 2 * wr + 1 <= W, "NCC window larger than the image");
+    RTS_CHECK(l1 > 0 && l2 > 0 && l3 > 0, "combine weights must be positive");
+    RTS_CHECK(dch == 8 || dch == 16 || dch == 32, "disparities per block %d must be 8, 16 or 32", dch);
+    RTS_CHECK(div_up(D, dch) <= 65535 && div_up(H, 4) <= 65535, "image too large");
+    dim3 grid(div_up(W, 64), div_up(H, 4), div_up(D, dch)), block(64, 4);
+    hipStream_t s = S(stream);
+    if (dch == 8)
+        k_cost_rank_log_ncc<8><<<grid, block, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol, W,
+                                                      H, D, hr, wr, l1, l2, l3);
+    else if (dch == 16)
+        k_cost_rank_log_ncc<16><<<grid, block, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol, W,
+                                                       H, D, hr, wr, l1, l2, l3);
+    else
+        k_cost_rank_log_ncc<32><<<grid, block, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol, W,
+                                                       H, D, hr, wr, l1, l2, l3);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+rts_status rts_cost_rank_log_ncc(const uint8_t *gray_ref, const uint8_t *gray_tar, const uint8_t *rank_ref,
+                                 const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol, int W,
+                                 int H, int D, int hr, int wr, float l1, float l2, float l3, void *stream) {
+    return rts_cost_rank_log_ncc_dch(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol, W, H, D, hr, wr, l1,
+                                     l2, l3, 16, stream);
 }
 rts_status rts_combine(float *c1, const float *c2, float l1, float l2, size_t n, void *stream) {
     if (!n) return RTS_OK;

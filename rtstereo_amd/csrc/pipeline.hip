@@ -200,6 +200,9 @@ int validate(const rts_pipeline_desc &d) {
     RTS_CHECK(d.cross_iters >= 1 && d.cross_iters <= 16, "cross iterations %d out of range", d.cross_iters);
     RTS_CHECK(d.cross_L >= 0 && d.cross_L <= 255, "cross arm limit %d out of range", d.cross_L);
     RTS_CHECK(d.width <= FH_MAXW, "width %d exceeds %d", d.width, FH_MAXW);
+    RTS_CHECK(d.cost3 == 0 || (d.cost1 == RTS_COST_RANK && d.cost2 == RTS_COST_LOG && d.cost3 == RTS_COST_NCC),
+              "a third cost is only supported as rank + LoG + NCC");
+    RTS_CHECK(d.cost3 == 0 || (d.lambda1 > 0 && d.lambda2 > 0 && d.lambda3 > 0), "combine weights must be positive");
     return RTS_OK;
 }
 
@@ -237,6 +240,8 @@ void rts_pipeline_defaults(rts_pipeline_desc *d, int W, int H, int D) {
     d->wta_literal = 0;
     d->lr_check = 1;
     d->lr_tol = 0;
+    d->cost3 = 0;
+    d->lambda3 = 0.5f;
 }
 
 size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
@@ -412,6 +417,15 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
                                                                   p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
         RTS_LAUNCHED();
         prefixed = true;
+    } else if (d.cost3) {
+        for (int v = 0; v < 2; v++) {
+            if (int e = rts_rank(gray[v], p->rank[v], W, H, d.rank_m, s)) return e;
+            if (int e = rts_convolve(gray[v], p->logimg[v], kLoG5, W, H, 2, 2, s)) return e;
+        }
+        RTS_MARK(p, "prep", s);
+        if (int e = rts_cost_rank_log_ncc(gray[0], gray[1], p->rank[0], p->rank[1], p->logimg[0], p->logimg[1], cur, W,
+                                          H, D, d.ncc_hr, d.ncc_wr, d.lambda1, d.lambda2, d.lambda3, s))
+            return e;
     } else {
         RTS_MARK(p, "prep", s);
         if (int e = build_cost(p, d.cost1, rgb, gray, cur, s)) return e;

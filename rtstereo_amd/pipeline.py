@@ -31,6 +31,9 @@ def preset(name: str, W: int, H: int, D: int, **over) -> PipelineDesc:
     aw:           colour AD truncated at 40, adaptive weights 33x33 (gc 7, gp 36),
                   WTA, check tol 1
     rank_box / log_box / ncc_box: the remaining cost families with an 11x11 box
+    rank_log_ncc: the batched workload: rank 7x7 -> AD, LoG 5x5 -> AD and NCC
+                  7x7 combined as three exponential terms (lambda 8 / 10 / 0.5,
+                  NCC as 1 - ncc), 11x11 box, WTA, check tol 1
     """
     d = defaults(W, H, D)
     d.lr_check, d.lr_tol = 1, 1
@@ -50,6 +53,11 @@ def preset(name: str, W: int, H: int, D: int, **over) -> PipelineDesc:
         d.cost1, d.aggr, d.box_r = L.RTS_COST_LOG, L.RTS_AGGR_BOX, 5
     elif name == "ncc_box":
         d.cost1, d.ncc_hr, d.ncc_wr, d.aggr, d.box_r = L.RTS_COST_NCC, 3, 3, L.RTS_AGGR_BOX, 5
+    elif name == "rank_log_ncc":
+        d.cost1, d.cost2, d.cost3 = L.RTS_COST_RANK, L.RTS_COST_LOG, L.RTS_COST_NCC
+        d.rank_m, d.ncc_hr, d.ncc_wr = 3, 3, 3
+        d.lambda1, d.lambda2, d.lambda3 = 8.0, 10.0, 0.5
+        d.aggr, d.box_r = L.RTS_AGGR_BOX, 5
     else:
         raise ValueError(f"unknown preset {name!r}")
     for k, v in over.items():

@@ -1,0 +1,31 @@
+"""Batch engine argument checks that need no GPU."""
+import ctypes as C
+
+import pytest
+
+from rtstereo_amd import _lib as L
+from rtstereo_amd._lib import RtsError
+from rtstereo_amd import batch
+from rtstereo_amd.pipeline import preset
+
+
+def test_batch_rejects_other_configurations():
+    with pytest.raises(RtsError) as e:
+        batch.Batch(preset("census_cross", 64, 48, 16), 1 << 30)
+    assert e.value.status == L.RTS_ERR_INVALID
+    d = preset("rank_log_ncc", 64, 48, 16)
+    d.box_r = 9
+    with pytest.raises(RtsError):
+        batch.Batch(d, 1 << 30)
+
+
+def test_pair_bytes_and_budget():
+    W, H, D = 640, 480, 64
+    n = batch.pair_bytes(W, H, D)
+    assert n >= 2 * W * H * D * 4 + 12 * W * H
+    b = batch.Batch(preset("rank_log_ncc", W, H, D), n - 1)
+    fake = 4096   # never dereferenced: the run is refused before any device work
+    it = (batch.BatchItem * 1)(batch.BatchItem(W, H, D, fake, fake, fake, fake, fake))
+    with pytest.raises(RtsError):
+        L.call("rts_batch_run", b.handle, it, 1, None)
+    b.close()

@@ -29,9 +29,9 @@ def test_c_abi_exports_every_declared_function():
 
 
 def test_c_abi_binding_covers_header():
-    from rtstereo_amd import _lib, serve
+    from rtstereo_amd import _lib, serve, batch
     names = _declared_c_functions()
-    bound = set(_lib._SIGS) | set(serve._SIGS)
+    bound = set(_lib._SIGS) | set(serve._SIGS) | set(batch._SIGS)
     assert set(names) <= bound, set(names) - bound
 
 
