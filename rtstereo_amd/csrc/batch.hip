@@ -39,25 +39,32 @@ struct BoxPair {
 };
 
 // Work item t of pair p: d = t / (ny*nx), strip = (t / nx) % ny, tile = t % nx.
+constexpr int BQ_MAXP = 256;   // pairs per chunk
+
 template <int TY>
 __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ pairs, int npairs, unsigned ntiles,
                                                    unsigned *__restrict__ counter, int r) {
     __shared__ float tile[TY + 2 * BQ_RMAX][64 + 2 * BQ_RMAX + 1];
     __shared__ float hs[TY + 2 * BQ_RMAX][64];
+    __shared__ unsigned first[BQ_MAXP];   // each pair's first work item
     __shared__ unsigned item;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int e = 2 * r + 1;
     const float area = (float)(e * e);
+    for (int i = threadIdx.x; i < npairs; i += 256) first[i] = pairs[i].tile0;
+    // The next work item is requested while the current one is processed.
+    unsigned nxt = 0;
+    if (threadIdx.x == 0) nxt = atomicAdd(counter, 1u);
     for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(counter, 1u);
-        __syncthreads();
+        if (threadIdx.x == 0) item = nxt;
+        __syncthreads();   // also: the previous item's LDS reads are done
         const unsigned t = item;
-        __syncthreads();   // every thread has read the item before thread 0 takes the next
         if (t >= ntiles) break;
+        if (threadIdx.x == 0) nxt = atomicAdd(counter, 1u);
         int lo = 0, hi = npairs - 1;   // last pair whose first item is <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (pairs[mid].tile0 <= t) lo = mid;
+            if (first[mid] <= t) lo = mid;
             else hi = mid - 1;
         }
         const BoxPair bp = pairs[lo];
@@ -66,9 +73,27 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
         const int W = bp.W, H = bp.H, x0 = xb * 64, y0 = sb * TY;
         const size_t base = (size_t)d * plane(W, H);
         const int th = TY + 2 * r, tw = 64 + 2 * r;
-        for (int i = threadIdx.x; i < th * tw; i += 256) {
+        const bool inner = x0 - r >= 0 && x0 + 64 + r <= W && y0 - r >= 0 && y0 + TY + r <= H;
+        const float *src = bp.in + base;
+        // All of a thread's loads are issued before its LDS stores.
+        constexpr int NB = ((TY + 2 * BQ_RMAX) * (64 + 2 * BQ_RMAX) + 255) / 256;
+        float v[NB];
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int i = threadIdx.x + 256 * k;
             const int rr = i / tw, cc = i - rr * tw;
-            tile[rr][cc] = bp.in[base + (size_t)wrapn(y0 - r + rr, H) * W + wrapn(x0 - r + cc, W)];
+            float x = 0.0f;
+            if (rr < th) {
+                const int yy = y0 - r + rr, xx = x0 - r + cc;
+                x = inner ? src[(size_t)yy * W + xx] : src[(size_t)wrapn(yy, H) * W + wrapn(xx, W)];
+            }
+            v[k] = x;
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int i = threadIdx.x + 256 * k;
+            const int rr = i / tw, cc = i - rr * tw;
+            if (rr < th) tile[rr][cc] = v[k];
         }
         __syncthreads();
         for (int rr = ty; rr < th; rr += 4) {
@@ -363,7 +388,7 @@ rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void 
     size_t used = 0, arena_need = 0;
     for (int i = 0; i < n; i++) {
         const size_t pb = pair_bytes(items[i]);
-        if (plan.empty() || used + pb > b->budget) {
+        if (plan.empty() || used + pb > b->budget || plan.back().size() >= (size_t)BQ_MAXP) {
             plan.emplace_back();
             used = 0;
         }

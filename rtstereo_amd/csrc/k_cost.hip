@@ -275,14 +275,18 @@ __global__ void __launch_bounds__(256) k_ncc_int(const uint8_t *__restrict__ ref
 constexpr int NC_RMAX = 5;
 constexpr int NC_TH = 4 + 2 * NC_RMAX;   // staged rows
 
+// The k_ncc_int quotient with the denominator factored as N^4 den^2 = N * Z,
+//   Z = N*(N^2 Saabb - 2N(Sb Saab + Sa Sabb) + Sb^2 Saa + Sa^2 Sbb + 4 Sa Sb Sab) - 3 Sa^2 Sb^2,
+// Z = sum over taps of (N a - Sa)^2 (N b - Sb)^2 / N <= (255 N)^4: for
+// windows of up to 121 taps every partial term stays below 2^63, so Z is
+// exact in 64-bit integers and the flat test Z == 0 is exact.
 __device__ __forceinline__ float ncc_exact(long N, long Sa, long Saa, long Sb, long Sbb, long Sab, long Saab, long Sabb,
                                            unsigned long Saabb) {
-    const __int128 n = N;
-    const __int128 X = n * n * n * n * (__int128)Saabb - 2 * n * n * n * ((__int128)Sb * Saab + (__int128)Sa * Sabb) +
-                       n * n * ((__int128)Sb * Sb * Saa + (__int128)Sa * Sa * Sbb + 4 * (__int128)Sa * Sb * Sab) -
-                       3 * n * (__int128)Sa * Sa * Sb * Sb;
+    const long Y = N * N * (long)Saabb - 2 * N * (Sb * Saab + Sa * Sabb) + Sb * Sb * Saa + Sa * Sa * Sbb +
+                   4 * Sa * Sb * Sab;
+    const long Z = N * Y - 3 * (Sa * Sb) * (Sa * Sb);
     const long nomN = N * Sab - Sa * Sb;
-    return X > 0 ? (float)((double)N * (double)nomN / sqrt((double)X)) : 0.0f;
+    return Z > 0 ? (float)((double)N * (double)nomN / sqrt((double)N * (double)Z)) : 0.0f;
 }
 
 template <int NC_DCH>
@@ -344,6 +348,120 @@ __global__ void __launch_bounds__(256) k_cost_rank_log_ncc(const uint8_t *__rest
         const float t2 = 1.0f - expf(__fdiv_rn(-cl, l2));
         const float t3 = 1.0f - expf(__fdiv_rn(-(1.0f - s), l3));
         vol[d * pl + (size_t)y * W + x] = (t1 + t2) + t3;
+    }
+}
+
+// The same cost for a (2HR+1) x (2WR+1) NCC window by a row sweep.  Block:
+// 64 columns x a segment of NS_SEG output rows, four waves each on its own
+// disparity, DCH/4 disparity rounds.  Per row each wave stages (a, b) of the
+// columns its lanes' windows touch as 16-bit pairs in LDS and forms the
+// row's horizontal window sums of a, a^2, b, b^2, ab, a^2b, ab^2, a^2b^2;
+// the last 2HR+1 of them stay in a register ring, so the vertical sums move
+// by one add and one subtract per row.  The sums are the exact integers of
+// k_cost_rank_log_ncc, so both kernels write identical values.
+constexpr int NS_SEG = 64;
+
+struct NccSums {
+    int sa, saa, sb, sbb, sab, saab, sabb;
+    unsigned long saabb;
+};
+
+__device__ __forceinline__ void ncc_add(NccSums &s, const NccSums &h, const NccSums &o) {
+    s.sa += h.sa - o.sa;
+    s.saa += h.saa - o.saa;
+    s.sb += h.sb - o.sb;
+    s.sbb += h.sbb - o.sbb;
+    s.sab += h.sab - o.sab;
+    s.saab += h.saab - o.saab;
+    s.sabb += h.sabb - o.sabb;
+    s.saabb += h.saabb - o.saabb;
+}
+
+// ncc_exact with the quotient in float (the integers Z and N*nom are exact;
+// the float rounding of the quotient is ~1e-7 relative).
+__device__ __forceinline__ float ncc_fast(long N, long Sa, long Saa, long Sb, long Sbb, long Sab, long Saab, long Sabb,
+                                          unsigned long Saabb) {
+    const long Y = N * N * (long)Saabb - 2 * N * (Sb * Saab + Sa * Sabb) + Sb * Sb * Saa + Sa * Sa * Sbb +
+                   4 * Sa * Sb * Sab;
+    const long Z = N * Y - 3 * (Sa * Sb) * (Sa * Sb);
+    const long nomN = N * Sab - Sa * Sb;
+    return Z > 0 ? (float)N * (float)nomN * __builtin_amdgcn_rsqf((float)N * (float)Z) : 0.0f;
+}
+
+template <int HR, int WR, int DCH>
+__global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restrict__ gl, const uint8_t *__restrict__ gr,
+                                                        const uint8_t *__restrict__ rkl, const uint8_t *__restrict__ rkr,
+                                                        const float *__restrict__ lgl, const float *__restrict__ lgr,
+                                                        float *__restrict__ vol, int W, int H, int D, float l1,
+                                                        float l2, float l3) {
+    constexpr int E = 2 * HR + 1, CW = 64 + 2 * WR;
+    constexpr long N = (long)E * (2 * WR + 1);
+    __shared__ uint16_t buf[4][CW];
+    const float k1 = -1.4426950408889634f / l1, k2 = -1.4426950408889634f / l2, k3 = -1.4426950408889634f / l3;
+    const int x0 = blockIdx.x * 64, s0 = blockIdx.y * NS_SEG, s1 = min(H, s0 + NS_SEG);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int x = x0 + lane;
+    const size_t pl = plane(W, H);
+    const int nr = (s1 - s0) + 2 * HR;   // rows swept (the same for every wave)
+    for (int round = 0; round < DCH / 4; round++) {
+        const int d = blockIdx.z * DCH + round * 4 + w;
+        const bool dok = d < D;
+        const int dd = dok ? d : 0;
+        NccSums ring[E], S = {};
+#pragma unroll
+        for (int k = 0; k < E; k++) ring[k] = NccSums{};
+        for (int base = 0; base < nr; base += E) {
+#pragma unroll
+            for (int k = 0; k < E; k++) {
+                const int i = base + k;
+                if (i >= nr) break;
+                const int y = wrapn(s0 - HR + i, H);
+                // Each wave works on its own LDS row: a wave-level fence and
+                // barrier order its previous reads before these writes, and
+                // these writes before the reads below.
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (int c = lane; c < CW; c += 64) {
+                    int xa = x0 - WR + c, xb = xa - dd;
+                    xa = xa < 0 ? xa + W : (xa >= W ? xa - W : xa);
+                    xb = xb < 0 ? xb + W : (xb >= W ? xb - W : xb);
+                    if (xa < 0 || xa >= W) xa = wrapn(xa, W);   // images narrower than the window reach
+                    if (xb < 0 || xb >= W) xb = wrapn(xb, W);
+                    buf[w][c] = (uint16_t)(gl[(size_t)y * W + xa] | (gr[(size_t)y * W + xb] << 8));
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                NccSums h = {};
+#pragma unroll
+                for (int dx = 0; dx <= 2 * WR; dx++) {
+                    const int v = buf[w][lane + dx];
+                    const int a = v & 255, b = v >> 8, ab = a * b;
+                    h.sa += a;
+                    h.saa += a * a;
+                    h.sb += b;
+                    h.sbb += b * b;
+                    h.sab += ab;
+                    h.saab += a * ab;
+                    h.sabb += b * ab;
+                    h.saabb += (unsigned long)((unsigned)ab * (unsigned)ab);
+                }
+                ncc_add(S, h, ring[k]);
+                ring[k] = h;
+                if (i >= 2 * HR && dok && x < W) {
+                    const int yo = s0 + i - 2 * HR;
+                    const float sn = ncc_fast(N, S.sa, S.saa, S.sb, S.sbb, S.sab, S.saab, S.sabb, S.saabb);
+                    const size_t o = (size_t)yo * W;
+                    const int xt = wrap1(x - d, W);
+                    const float cr = fabsf((float)rkl[o + x] - (float)rkr[o + xt]);
+                    const float cl = fabsf(lgl[o + x] - lgr[o + xt]);
+                    // exp(-c/l) = exp2(c * k), k = -log2(e)/l (hardware exp2)
+                    const float t1 = 1.0f - __builtin_amdgcn_exp2f(cr * k1);
+                    const float t2 = 1.0f - __builtin_amdgcn_exp2f(cl * k2);
+                    const float t3 = 1.0f - __builtin_amdgcn_exp2f((1.0f - sn) * k3);
+                    vol[d * pl + o + x] = (t1 + t2) + t3;
+                }
+            }
+        }
     }
 }
 
@@ -470,8 +588,22 @@ rts_status rts_cost_rank_log_ncc_dch(const uint8_t *gray_ref, const uint8_t *gra
     RTS_CHECK(l1 > 0 && l2 > 0 && l3 > 0, "combine weights must be positive");
     RTS_CHECK(dch == 8 || dch == 16 || dch == 32, "disparities per block %d must be 8, 16 or 32", dch);
     RTS_CHECK(div_up(D, dch) <= 65535 && div_up(H, 4) <= 65535, "image too large");
-    dim3 grid(div_up(W, 64), div_up(H, 4), div_up(D, dch)), block(64, 4);
     hipStream_t s = S(stream);
+    if (hr == 3 && wr == 3) {   // the batched workload's window: row sweep
+        dim3 grid(div_up(W, 64), div_up(H, NS_SEG), div_up(D, dch));
+        if (dch == 8)
+            k_cost_rln_sweep<3, 3, 8><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol,
+                                                            W, H, D, l1, l2, l3);
+        else if (dch == 16)
+            k_cost_rln_sweep<3, 3, 16><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar,
+                                                             vol, W, H, D, l1, l2, l3);
+        else
+            k_cost_rln_sweep<3, 3, 32><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar,
+                                                             vol, W, H, D, l1, l2, l3);
+        RTS_LAUNCHED();
+        return RTS_OK;
+    }
+    dim3 grid(div_up(W, 64), div_up(H, 4), div_up(D, dch)), block(64, 4);
     if (dch == 8)
         k_cost_rank_log_ncc<8><<<grid, block, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol, W,
                                                       H, D, hr, wr, l1, l2, l3);
