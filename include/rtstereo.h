@@ -305,8 +305,9 @@ size_t rts_pipeline_bytes(const rts_pipeline_desc *desc);
 rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out);
 rts_status rts_pipeline_destroy(rts_pipeline *p);
 /* Runs on device images: colour [H][W][3] each; gray [H][W] is derived on
- * the device when left_gray/right_gray are NULL; Lab [H][W][3] must be given
- * for adaptive-weight aggregation.  dl is required, dr and mask when lr_check. */
+ * the device when left_gray/right_gray are NULL, and so is Lab [H][W][3]
+ * (rts_rgb_to_lab) for adaptive-weight aggregation when left_lab/right_lab
+ * are NULL.  dl is required, dr and mask when lr_check. */
 rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
                             const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
                             const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask, void *stream);
@@ -399,6 +400,10 @@ rts_status rts_server_wait(rts_server *s, uint64_t ticket, rts_request_stats *st
 rts_status rts_server_cancel(rts_server *s, uint64_t ticket);
 /* Stops or resumes admission (queued requests wait while paused). */
 rts_status rts_server_pause(rts_server *s, int paused);
+/* Frames of camera streams (stream_id >= 0) are admitted before one-off
+ * requests; at most `limit` one-off requests run at once (default: all
+ * slots), which keeps slots free for the streams. */
+rts_status rts_server_set_oneoff_limit(rts_server *s, int limit);
 rts_status rts_server_close_stream(rts_server *s, int stream_id);
 rts_status rts_server_counters(rts_server *s, rts_server_stats *st);
 
@@ -406,6 +411,14 @@ rts_status rts_server_counters(rts_server *s, rts_server_stats *st);
  * OpenCV's imread, seq/Main.cpp:72-73): gray = (29B + 150G + 77R + 128) >> 8
  * on interleaved B,G,R bytes. */
 rts_status rts_rgb_to_gray(const uint8_t *rgb, uint8_t *gray, int W, int H, void *stream);
+/* 8-bit CIELab (D65) from B,G,R bytes, for the adaptive weights (the
+ * reference takes Lab from OpenCV's cvtColor, seq/Main.cpp:96-97): sRGB
+ * linearised, XYZ, then L*255/100, a+128, b+128 rounded and clipped, in FP64. */
+rts_status rts_rgb_to_lab(const uint8_t *rgb, uint8_t *lab, int W, int H, void *stream);
+/* 3x3 per-channel median with replicated borders on B,G,R bytes, out of
+ * place: the OpenCL driver's pre-filter before cross arms
+ * (opencl/Main.cpp:342-345). */
+rts_status rts_median3(const uint8_t *rgb, uint8_t *out, int W, int H, void *stream);
 
 #ifdef __cplusplus
 }

@@ -237,6 +237,15 @@ def aw_rows_f64(labL, labR, cost, rows, r, gc, gp):
     return out
 
 
+def median3(rgb):
+    """3x3 median per channel with replicated borders (opencl/Main.cpp:342-345
+    applies OpenCV's medianBlur with ksize 3 before the cross arms)."""
+    H, W, _ = rgb.shape
+    p = np.pad(rgb, ((1, 1), (1, 1), (0, 0)), mode="edge")
+    stack = np.stack([p[dy:dy + H, dx:dx + W] for dy in range(3) for dx in range(3)], 0)
+    return np.sort(stack, 0)[4].astype(np.uint8)
+
+
 def find_cross(rgb, L=17, tao=20):
     H, W, _ = rgb.shape
     horz = np.zeros((H, W, 2), np.int32)

@@ -99,6 +99,8 @@ _SIGS = {
     "rts_pipeline_enable_timing": [vp, i32],
     "rts_pipeline_stage_ms": [vp, C.POINTER(f32), C.POINTER(cp), i32, C.POINTER(i32)],
     "rts_rgb_to_gray": [vp, vp, i32, i32, vp],
+    "rts_rgb_to_lab": [vp, vp, i32, i32, vp],
+    "rts_median3": [vp, vp, i32, i32, vp],
 }
 _RESTYPES = {"rts_last_error": cp, "rts_version": cp, "rts_pipeline_defaults": None,
              "rts_pipeline_bytes": sz, "rts_pipeline_volume": vp,

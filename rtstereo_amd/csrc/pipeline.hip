@@ -143,6 +143,7 @@ struct rts_pipeline {
     // volumes
     float *vol[2] = {nullptr, nullptr};
     float *final_vol = nullptr;
+    uint8_t *lab[2] = {nullptr, nullptr};   // Lab images when the caller passes none
     void *aw_ws = nullptr;   // adaptive-weight band workspace
     size_t aw_ws_bytes = 0;
     // tables for the fused front end
@@ -252,7 +253,7 @@ size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
     if (d->aggr == RTS_AGGR_CROSS) b += 2 * P * 4;
     if (uses(*d, RTS_COST_RANK)) b += 2 * P;
     if (uses(*d, RTS_COST_LOG)) b += 2 * P * 4;
-    if (d->aggr == RTS_AGGR_AW) b += rts_aw_workspace_bytes(d->width, d->height, d->aw_r, d->aw_precision);
+    if (d->aggr == RTS_AGGR_AW) b += rts_aw_workspace_bytes(d->width, d->height, d->aw_r, d->aw_precision) + 6 * P;
     return b + 8192;
 }
 
@@ -281,6 +282,7 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
     if (d.aggr == RTS_AGGR_AW) {
         p->aw_ws_bytes = rts_aw_workspace_bytes(d.width, d.height, d.aw_r, d.aw_precision);
         A(&p->aw_ws, p->aw_ws_bytes);
+        for (int v = 0; v < 2; v++) A((void **)&p->lab[v], 3 * P);
     }
     // Fused front end: census and/or colour AD (combined) feeding cross-based aggregation.
     const bool front_ok = d.aggr == RTS_AGGR_CROSS && d.width >= 8 &&
@@ -325,6 +327,7 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
         (void)hipFree(p->vol[v]);
     }
     (void)hipFree(p->aw_ws);
+    for (int v = 0; v < 2; v++) (void)hipFree(p->lab[v]);
     (void)hipFree(p->tabC);
     (void)hipFree(p->tabA);
     for (int i = 0; i < 8; i++)
@@ -378,7 +381,7 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
     RTS_CHECK(left_rgb && right_rgb, "colour images are required");
     RTS_CHECK(dl != nullptr, "dl is required");
     RTS_CHECK(!d.lr_check || (dr && mask), "lr_check needs dr and mask");
-    RTS_CHECK(d.aggr != RTS_AGGR_AW || (left_lab && right_lab), "adaptive weights need Lab images");
+    RTS_CHECK(!left_lab == !right_lab, "give both Lab images or neither");
     const uint8_t *rgb[2] = {left_rgb, right_rgb};
     const uint8_t *gray[2] = {left_gray, right_gray};
     const bool need_gray = uses(d, RTS_COST_CENSUS) || uses(d, RTS_COST_RANK) || uses(d, RTS_COST_LOG) ||
@@ -390,6 +393,12 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
         }
     p->nmarks = 0;
     RTS_MARK(p, "start", s);
+    if (d.aggr == RTS_AGGR_AW && !left_lab) {   // Lab on the device
+        for (int v = 0; v < 2; v++)
+            if (int e = rts_rgb_to_lab(rgb[v], p->lab[v], W, H, s)) return e;
+        left_lab = p->lab[0];
+        right_lab = p->lab[1];
+    }
     float *cur = p->vol[0], *other = p->vol[1];
     if (d.aggr == RTS_AGGR_CROSS)
         for (int v = 0; v < 2; v++)

@@ -3,6 +3,10 @@
 // the disparity-major device layout, and small utility kernels.
 #include "rts_internal.h"
 
+#include <math.h>
+
+#include <mutex>
+
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -62,6 +66,58 @@ __global__ void k_rgb_to_gray(const uint8_t *__restrict__ rgb, uint8_t *__restri
     if (i >= n) return;
     const uint32_t b = rgb[3 * i], g = rgb[3 * i + 1], r = rgb[3 * i + 2];
     gray[i] = (uint8_t)((29u * b + 150u * g + 77u * r + 128u) >> 8);
+}
+
+// 8-bit CIELab (D65) from interleaved B,G,R bytes, in FP64: L*255/100,
+// a+128, b+128, rounded half to even and clipped.  srgb_lin[v] is the
+// linearised channel value v/255 (host table).
+__global__ void k_rgb_to_lab(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ lab,
+                             const double *__restrict__ srgb_lin, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double b = srgb_lin[rgb[3 * i]], g = srgb_lin[rgb[3 * i + 1]], r = srgb_lin[rgb[3 * i + 2]];
+    double X = 0.4124564 * r + 0.3575761 * g + 0.1804375 * b;
+    double Y = 0.2126729 * r + 0.7151522 * g + 0.0721750 * b;
+    double Z = 0.0193339 * r + 0.1191920 * g + 0.9503041 * b;
+    X /= 0.950456;
+    Z /= 1.088754;
+    const double eps = 216.0 / 24389.0, kap = 24389.0 / 27.0;
+    const double fx = X > eps ? cbrt(X) : (kap * X + 16.0) / 116.0;
+    const double fy = Y > eps ? cbrt(Y) : (kap * Y + 16.0) / 116.0;
+    const double fz = Z > eps ? cbrt(Z) : (kap * Z + 16.0) / 116.0;
+    const double v[3] = {(116.0 * fy - 16.0) * 255.0 / 100.0, 500.0 * (fx - fy) + 128.0, 200.0 * (fy - fz) + 128.0};
+#pragma unroll
+    for (int c = 0; c < 3; c++) lab[3 * i + c] = (uint8_t)fmin(255.0, fmax(0.0, rint(v[c])));
+}
+
+// 3x3 median of each colour channel with replicated borders (the OpenCL
+// driver's pre-filter, opencl/Main.cpp:342-345, there from OpenCV's
+// medianBlur).  A 19-exchange sorting network on the nine values.
+__device__ __forceinline__ void cswap(int &a, int &b) {
+    const int lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
+__global__ void k_median3(const uint8_t *__restrict__ in, uint8_t *__restrict__ out, int W, int H) {
+    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    for (int c = 0; c < 3; c++) {
+        int v[9], k = 0;
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int yy = min(max(y + dy, 0), H - 1), xx = min(max(x + dx, 0), W - 1);
+                v[k++] = in[3 * ((size_t)yy * W + xx) + c];
+            }
+        cswap(v[1], v[2]); cswap(v[4], v[5]); cswap(v[7], v[8]);
+        cswap(v[0], v[1]); cswap(v[3], v[4]); cswap(v[6], v[7]);
+        cswap(v[1], v[2]); cswap(v[4], v[5]); cswap(v[7], v[8]);
+        cswap(v[0], v[3]); cswap(v[5], v[8]); cswap(v[4], v[7]);
+        cswap(v[3], v[6]); cswap(v[1], v[4]); cswap(v[2], v[5]);
+        cswap(v[4], v[7]); cswap(v[4], v[2]); cswap(v[6], v[4]);
+        cswap(v[4], v[2]);
+        out[3 * ((size_t)y * W + x) + c] = (uint8_t)v[4];
+    }
 }
 
 }  // namespace rts
@@ -200,6 +256,42 @@ rts_status rts_rgb_to_gray(const uint8_t *rgb, uint8_t *gray, int W, int H, void
     RTS_CHECK(W >= 1 && H >= 1, "bad image size");
     const int n = W * H;
     k_rgb_to_gray<<<div_up(n, 256), 256, 0, S(stream)>>>(rgb, gray, n);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+rts_status rts_rgb_to_lab(const uint8_t *rgb, uint8_t *lab, int W, int H, void *stream) {
+    RTS_CHECK(W >= 1 && H >= 1 && (long)W * H < (1L << 31), "bad image size %dx%d", W, H);
+    static double *table[64] = {};   // per device: the 256 linearised sRGB values
+    int dev = 0;
+    RTS_HIP(hipGetDevice(&dev));
+    RTS_CHECK(dev >= 0 && dev < 64, "device %d out of range", dev);
+    static std::mutex mu;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        if (!table[dev]) {
+            double h[256];
+            for (int v = 0; v < 256; v++) {
+                const double c = v / 255.0;
+                h[v] = c <= 0.04045 ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4);
+            }
+            double *d = nullptr;
+            RTS_HIP(hipMalloc((void **)&d, sizeof(h)));
+            RTS_HIP(hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice));
+            table[dev] = d;
+        }
+    }
+    const int n = W * H;
+    k_rgb_to_lab<<<div_up(n, 256), 256, 0, S(stream)>>>(rgb, lab, table[dev], n);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+rts_status rts_median3(const uint8_t *rgb, uint8_t *out, int W, int H, void *stream) {
+    RTS_CHECK(W >= 1 && H >= 1 && div_up(H, 4) <= 65535, "bad image size %dx%d", W, H);
+    RTS_CHECK(rgb != out, "rts_median3 is out of place");
+    dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
+    k_median3<<<grid, block, 0, S(stream)>>>(rgb, out, W, H);
     RTS_LAUNCHED();
     return RTS_OK;
 }

@@ -62,7 +62,7 @@ size_t io_bytes(const rts_pipeline_desc &d) {
 
 struct rts_server {
     size_t budget = 0, in_use = 0;
-    int nslots = 0;
+    int nslots = 0, oneoff_limit = 0;
     bool stop = false, paused = false;
     clk::time_point t0 = clk::now();
     uint64_t next_ticket = 1, use_clock = 0;
@@ -215,55 +215,64 @@ void worker_main(rts_server *s, int slot) {
     if (st) (void)hipStreamDestroy(st);
 }
 
-// Admission: requests in arrival order.  A request starts when a slot is
-// free and it either finds an idle resident pipeline of its configuration or
-// its footprint fits the budget once idle pipelines are evicted.  Once one
-// request is waiting for memory, the requests behind it may still start on
-// resident pipelines but may not allocate, so a large request is not starved
-// by a stream of smaller ones.
+// Admission, in two passes over the queue in arrival order: first the
+// frames of camera streams, then one-off requests, of which at most
+// `oneoff_limit` run at a time (so frames of resident streams always find a
+// slot).  A request starts when a slot is free and it either finds an idle
+// resident pipeline of its configuration or its footprint fits the budget
+// once idle pipelines are evicted.  Once a request of a pass is waiting for
+// memory, the requests behind it in that pass may still start on resident
+// pipelines but may not allocate, so a large request is not starved by a
+// stream of smaller ones.
 void dispatcher_main(rts_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (!s->stop) {
         bool progressed = false;
         if (!s->paused) {
-            bool blocked = false;
-            for (auto it = s->queue.begin(); it != s->queue.end();) {
-                int slot = -1;
-                for (int i = 0; i < s->nslots; i++)
-                    if (!s->assigned[i]) {
-                        slot = i;
-                        break;
+            for (int pass = 0; pass < 2; pass++) {
+                bool blocked = false;
+                for (auto it = s->queue.begin(); it != s->queue.end();) {
+                    int slot = -1, oneoff = 0;
+                    for (int i = 0; i < s->nslots; i++) {
+                        if (!s->assigned[i]) {
+                            if (slot < 0) slot = i;
+                        } else if (s->assigned[i]->req.stream_id < 0) {
+                            oneoff++;
+                        }
                     }
-                if (slot < 0) break;
-                std::shared_ptr<Request> r = *it;
-                if (stream_busy(s, r->req.stream_id)) {
-                    ++it;
-                    continue;
-                }
-                Resident *res = find_resident(s, *r);
-                if (!res) {
-                    if (blocked || s->in_use - idle_bytes(s) + r->footprint > s->budget) {
-                        blocked = true;
+                    if (slot < 0) break;
+                    std::shared_ptr<Request> r = *it;
+                    const bool is_stream = r->req.stream_id >= 0;
+                    if (is_stream != (pass == 0) || (!is_stream && oneoff >= s->oneoff_limit) ||
+                        stream_busy(s, r->req.stream_id)) {
                         ++it;
                         continue;
                     }
-                    evict_for(s, r->footprint);
-                    if (int e = make_resident(s, r->req.pipe, r->req.stream_id, &res)) {
-                        r->st.status = e;
-                        r->done = true;
-                        it = s->queue.erase(it);
-                        s->cv_done.notify_all();
-                        continue;
+                    Resident *res = find_resident(s, *r);
+                    if (!res) {
+                        if (blocked || s->in_use - idle_bytes(s) + r->footprint > s->budget) {
+                            blocked = true;
+                            ++it;
+                            continue;
+                        }
+                        evict_for(s, r->footprint);
+                        if (int e = make_resident(s, r->req.pipe, r->req.stream_id, &res)) {
+                            r->st.status = e;
+                            r->done = true;
+                            it = s->queue.erase(it);
+                            s->cv_done.notify_all();
+                            continue;
+                        }
+                        s->in_use += res->footprint;
                     }
-                    s->in_use += res->footprint;
+                    res->busy = true;
+                    r->started = true;
+                    r->st.footprint = res->footprint;
+                    s->assigned[slot] = r;
+                    s->assigned_res[slot] = res;
+                    it = s->queue.erase(it);
+                    progressed = true;
                 }
-                res->busy = true;
-                r->started = true;
-                r->st.footprint = res->footprint;
-                s->assigned[slot] = r;
-                s->assigned_res[slot] = res;
-                it = s->queue.erase(it);
-                progressed = true;
             }
         }
         if (progressed) s->cv_work.notify_all();
@@ -282,6 +291,7 @@ rts_status rts_server_create(size_t budget_bytes, int slots, rts_server **out) {
     rts_server *s = new rts_server();
     s->budget = budget_bytes;
     s->nslots = slots;
+    s->oneoff_limit = slots;
     s->assigned.assign(slots, nullptr);
     s->assigned_res.assign(slots, nullptr);
     for (int i = 0; i < slots; i++) s->workers.emplace_back(worker_main, s, i);
@@ -318,7 +328,7 @@ rts_status rts_server_submit(rts_server *s, const rts_request_desc *req, uint64_
     RTS_CHECK(s && req && ticket, "null argument");
     RTS_CHECK(req->left_rgb && req->right_rgb && req->dl, "request needs both colour images and dl");
     RTS_CHECK(!req->pipe.lr_check || (req->dr && req->mask), "lr_check needs dr and mask");
-    RTS_CHECK(req->pipe.aggr != RTS_AGGR_AW || (req->left_lab && req->right_lab), "AW requests need Lab images");
+    RTS_CHECK(!req->left_lab == !req->right_lab, "give both Lab images or neither");
     if (int e = rts::check_dims(req->pipe.width, req->pipe.height, req->pipe.disp)) return e;
     auto r = std::make_shared<Request>();
     r->req = *req;
@@ -372,6 +382,17 @@ rts_status rts_server_cancel(rts_server *s, uint64_t ticket) {
         }
     return rts::fail(RTS_ERR_INVALID, "ticket %llu is not waiting (running, finished or unknown)",
                      (unsigned long long)ticket);
+}
+
+rts_status rts_server_set_oneoff_limit(rts_server *s, int limit) {
+    RTS_CHECK(s != nullptr, "null server");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        RTS_CHECK(limit >= 1 && limit <= s->nslots, "one-off limit %d must be in [1, %d]", limit, s->nslots);
+        s->oneoff_limit = limit;
+    }
+    s->cv_work.notify_all();
+    return RTS_OK;
 }
 
 rts_status rts_server_pause(rts_server *s, int paused) {

@@ -43,6 +43,7 @@ _SIGS = {
     "rts_server_wait": ([C.c_void_p, C.c_uint64, C.POINTER(RequestStats)], C.c_int),
     "rts_server_cancel": ([C.c_void_p, C.c_uint64], C.c_int),
     "rts_server_pause": ([C.c_void_p, C.c_int], C.c_int),
+    "rts_server_set_oneoff_limit": ([C.c_void_p, C.c_int], C.c_int),
     "rts_server_close_stream": ([C.c_void_p, C.c_int], C.c_int),
     "rts_server_counters": ([C.c_void_p, C.POINTER(ServerStats)], C.c_int),
 }
@@ -87,8 +88,11 @@ class Server:
         req.stream_id = stream_id
         req.left_rgb, req.right_rgb = left.ctypes.data, right.ctypes.data
         if left_lab is not None:
-            left_lab, right_lab = np.ascontiguousarray(left_lab), np.ascontiguousarray(right_lab)
-            req.left_lab, req.right_lab = left_lab.ctypes.data, right_lab.ctypes.data
+            left_lab = np.ascontiguousarray(left_lab)
+            req.left_lab = left_lab.ctypes.data
+        if right_lab is not None:
+            right_lab = np.ascontiguousarray(right_lab)
+            req.right_lab = right_lab.ctypes.data
         req.dl, req.dr, req.mask = dl.ctypes.data, dr.ctypes.data, mask.ctypes.data
         t = C.c_uint64()
         call("rts_server_submit", self.handle, C.byref(req), C.byref(t))
@@ -104,6 +108,9 @@ class Server:
 
     def cancel(self, ticket: Ticket) -> None:
         call("rts_server_cancel", self.handle, ticket.id)
+
+    def set_oneoff_limit(self, limit: int) -> None:
+        call("rts_server_set_oneoff_limit", self.handle, limit)
 
     def pause(self, paused: bool = True) -> None:
         call("rts_server_pause", self.handle, int(paused))
@@ -215,3 +222,74 @@ def summarize(tickets, total_wall):
                          "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
                          "pd_per_s": pd / total_wall}
     return out
+
+
+def main(argv=None) -> int:
+    """Serving measurement: cameras at a fixed frame rate plus Poisson
+    adaptive-weight bursts, replayed in real time against one server.
+    Prints one JSON object with throughput, latency percentiles per request
+    kind and the agreement of served results with the same pairs run alone
+    (cameras: identical maps; bursts: the FP16 form against the plain FP32
+    per-frame path)."""
+    import argparse
+    import json
+    from . import pipeline as PL
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cameras", type=int, default=8)
+    ap.add_argument("--fps", type=float, default=30.0)
+    ap.add_argument("--burst-rate", type=float, default=40.0)
+    ap.add_argument("--duration", type=float, default=3.0)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--oneoff-limit", type=int, default=1, help="slots one-off bursts may hold at once")
+    ap.add_argument("--frames", type=int, default=2, help="distinct frames per camera (cycled)")
+    ap.add_argument("--burst-pool", type=int, default=6)
+    ap.add_argument("--seed", type=int, default=100)
+    args = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    cams, bursts = make_inputs(args.cameras, args.frames, args.burst_pool, args.seed)
+    gen_s = time.perf_counter() - t0
+    srv = Server(int(args.budget_gb * 2**30), args.slots)
+    srv.set_oneoff_limit(min(args.oneoff_limit, args.slots))
+    # warm-up: one frame per camera and one of each burst, so that resident
+    # pipelines exist and first-use costs stay out of the measurement
+    warm = [srv.submit(camera_desc(), c[0].left, c[0].right, stream_id=i, kind="camera") for i, c in enumerate(cams)]
+    for p, (la, lb) in bursts:
+        W, H = p.left.shape[1], p.left.shape[0]
+        warm.append(srv.submit(burst_desc(W, H, p.dmax), p.left, p.right, la, lb, kind="burst"))
+    for t in warm:
+        srv.wait(t)
+    tickets, submit_wall, total_wall = simulate(srv, cams, bursts, args.duration, args.fps, args.burst_rate, args.seed)
+    out = summarize(tickets, total_wall)
+    out["offered"] = {"cameras": args.cameras, "fps": args.fps, "burst_rate": args.burst_rate,
+                      "duration_s": args.duration, "submit_wall_s": submit_wall}
+    # agreement with each pair run alone
+    alone_cam, alone_burst = {}, {}
+    cam_same, burst_agree = [], []
+    for t in tickets:
+        key = t.keep[0].ctypes.data
+        if t.kind == "camera":
+            if key not in alone_cam:
+                alone_cam[key] = PL.run_host(camera_desc(), t.keep[0], t.keep[1])[0]
+            cam_same.append(bool(np.array_equal(t.dl, alone_cam[key])))
+        else:
+            if key not in alone_burst:
+                W, H, D = t.shape
+                alone_burst[key] = PL.run_host(burst_desc(W, H, D, L.RTS_AW_FP32), t.keep[0], t.keep[1],
+                                               t.keep[2], t.keep[3])[0]
+            burst_agree.append(float((t.dl == alone_burst[key]).mean()))
+    out["camera_identical_to_alone"] = float(np.mean(cam_same)) if cam_same else None
+    out["burst_fp16_agreement_with_fp32_alone"] = {"mean": float(np.mean(burst_agree)),
+                                                   "min": float(np.min(burst_agree))} if burst_agree else None
+    c = srv.counters()
+    out["server"] = {"budget_bytes": c.budget_bytes, "in_use_bytes": c.in_use_bytes, "resident": c.resident,
+                     "slots": args.slots, "oneoff_limit": min(args.oneoff_limit, args.slots)}
+    out["input_generation_s"] = gen_s
+    srv.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

@@ -223,3 +223,17 @@ def rgb_to_gray(rgb):
     a, g = DeviceArray.from_host(rgb), DeviceArray((H, W), np.uint8)
     call("rts_rgb_to_gray", a.ptr, g.ptr, W, H, None)
     return g.download()
+
+
+def rgb_to_lab(rgb):
+    H, W, _ = rgb.shape
+    a, o = DeviceArray.from_host(rgb), DeviceArray((H, W, 3), np.uint8)
+    call("rts_rgb_to_lab", a.ptr, o.ptr, W, H, None)
+    return o.download()
+
+
+def median3(rgb):
+    H, W, _ = rgb.shape
+    a, o = DeviceArray.from_host(rgb), DeviceArray((H, W, 3), np.uint8)
+    call("rts_median3", a.ptr, o.ptr, W, H, None)
+    return o.download()

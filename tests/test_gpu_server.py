@@ -103,3 +103,23 @@ def test_camera_stream_frames_in_order_and_resident():
     s.wait(t)
     assert s.counters().resident == 1
     s.close()
+
+
+def test_camera_frames_admitted_before_bursts():
+    """With one slot for one-off requests, a camera frame submitted after
+    three queued bursts starts before the second burst."""
+    print("[case] server priority: 3 bursts queued, then a camera frame", flush=True)
+    cams, bursts = _mix(81)
+    s = serve.Server(budget_bytes=8 << 30, slots=2)
+    s.set_oneoff_limit(1)
+    s.pause(True)
+    bd = PL.preset("census_cross", 160, 96, 24)
+    bt = [s.submit(bd, bursts[0].left, bursts[0].right, kind="burst") for _ in range(3)]
+    cd = PL.preset("census_cross", 192, 128, 32)
+    ct = s.submit(cd, cams[0].left, cams[0].right, stream_id=3, kind="camera")
+    s.pause(False)
+    for t in bt + [ct]:
+        s.wait(t)
+    assert ct.stats.start_ms <= bt[1].stats.start_ms
+    assert bt[1].stats.start_ms >= bt[0].stats.done_ms - 1e-6      # one one-off request at a time
+    s.close()

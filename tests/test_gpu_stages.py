@@ -17,6 +17,7 @@ pytestmark = pytest.mark.gpu
 
 from rtstereo_amd import stages as S  # noqa: E402  (loads the HIP library; fails loudly if missing)
 from rtstereo_amd import _lib as L  # noqa: E402
+from rtstereo_amd import pipeline as PL  # noqa: E402
 
 SIZES = [(40, 24, 8, 7), (29, 21, 5, 8), (67, 41, 13, 9), (130, 70, 64, 10)]
 
@@ -264,3 +265,22 @@ def test_errors_are_reported():
         S.ad_gray(np.zeros((4, 4), np.uint8), np.zeros((4, 4), np.uint8), 300)   # D > 256
     with pytest.raises(RtsError):
         S.box(np.zeros((2, 4, 4), np.float32), 9)                                # radius > image
+
+
+@pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
+def test_ingest_median_and_lab(W, H, D, seed):
+    p = _pair(W, H, D, seed)
+    assert np.array_equal(S.median3(p.left), oracle.median3(p.left))
+    lab = S.rgb_to_lab(p.left)
+    ref = gen.to_lab(p.left)
+    diff = np.abs(lab.astype(int) - ref.astype(int))
+    assert diff.max() <= 1 and (diff == 0).mean() >= 0.999
+
+
+def test_pipeline_computes_lab_on_device():
+    p = _pair(96, 64, 16, 9)
+    desc = PL.preset("aw", 96, 64, 16, aw_r=5)
+    la, lb = gen.to_lab(p.left), gen.to_lab(p.right)
+    dl1 = PL.run_host(desc, p.left, p.right, la, lb)[0]
+    dl2 = PL.run_host(desc, p.left, p.right)[0]
+    assert (dl1 == dl2).mean() >= 0.995

@@ -383,3 +383,15 @@ def test_math_refine_vote():
                 for v in disp[yy, lo:hi + 1]:
                     votes[v] += 1
             assert out[y, x] == int(np.argmax(votes))
+
+
+def test_median3_brute_force():
+    """The pre-filter oracle against the definition, pixel by pixel."""
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 256, (7, 9, 3), dtype=np.uint8)
+    m = oracle.median3(a)
+    for y in range(7):
+        for x in range(9):
+            for c in range(3):
+                v = [a[min(max(y + dy, 0), 6), min(max(x + dx, 0), 8), c] for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+                assert sorted(v)[4] == m[y, x, c]

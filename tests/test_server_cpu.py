@@ -29,8 +29,9 @@ def test_invalid_request_rejected():
     with pytest.raises(RtsError) as e:
         s.submit(preset("census_cross", 16, 16, 300), a, b)    # D > 256
     assert e.value.status == L.RTS_ERR_INVALID
+    a, b = _imgs(64, 48)
     with pytest.raises(RtsError):
-        s.submit(preset("aw", 64, 48, 16), *_imgs(64, 48))      # AW without Lab images
+        s.submit(preset("aw", 64, 48, 16), a, b, left_lab=a, right_lab=None)   # one Lab image only
     s.close()
 
 
@@ -54,3 +55,13 @@ def test_destroy_cancels_waiting_requests():
     a, b = _imgs(64, 48)
     s.submit(preset("census_cross", 64, 48, 16), a, b)
     s.close()        # must not hang with a queued request
+
+
+def test_oneoff_limit_range():
+    s = serve.Server(budget_bytes=1 << 30, slots=2)
+    with pytest.raises(RtsError):
+        s.set_oneoff_limit(0)
+    with pytest.raises(RtsError):
+        s.set_oneoff_limit(3)
+    s.set_oneoff_limit(2)
+    s.close()
