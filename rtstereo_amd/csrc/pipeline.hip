@@ -26,7 +26,8 @@ namespace rts {
 // One block per image row, one lane per disparity: lane d walks x = 0..W-1
 // serially, so its additions happen in the reference's order.  The right
 // view's census and colour rows sit in LDS (lanes read consecutive x-d);
-// the left view's values at x are uniform across the block.  Each lane
+// so do the left view's, whose values at x are the same for every lane
+// (broadcast reads).  Each lane
 // stages FH_CH prefix values in LDS, then the wave stores its 64 rows of
 // FH_CH floats contiguously.
 constexpr int FH_CH = 32;
@@ -38,16 +39,24 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
                                                      const float *__restrict__ tabC, const float *__restrict__ tabA,
                                                      int nA, float *__restrict__ P, int W, int H, int D, int vec) {
     extern __shared__ uint8_t smem[];
-    uint64_t *rowC = reinterpret_cast<uint64_t *>(smem);                        // W (census)
-    uint32_t *rowA = reinterpret_cast<uint32_t *>(rowC + (CENSUS ? W : 0));   // W (packed colour)
-    float *tC = reinterpret_cast<float *>(rowA + (AD ? W : 0));               // 65
+    uint64_t *rowC = reinterpret_cast<uint64_t *>(smem);                        // W (right census)
+    uint64_t *rowCL = rowC + (CENSUS ? W : 0);                                 // W (left census)
+    uint32_t *rowA = reinterpret_cast<uint32_t *>(rowCL + (CENSUS ? W : 0));  // W (right packed colour)
+    uint32_t *rowAL = rowA + (AD ? W : 0);                                     // W (left packed colour)
+    float *tC = reinterpret_cast<float *>(rowAL + (AD ? W : 0));              // 65
     float *tA = tC + 65;                                                       // nA
     float *stage = tA + nA;                                                    // waves * 64 * (FH_CH+1)
     const int y = blockIdx.x;
     const size_t roff = (size_t)y * W;
     for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        if (CENSUS) rowC[x] = cR[roff + x];
-        if (AD) rowA[x] = pack_rgb(rgbR + 3 * (roff + x));
+        if (CENSUS) {
+            rowC[x] = cR[roff + x];
+            rowCL[x] = cL[roff + x];
+        }
+        if (AD) {
+            rowA[x] = pack_rgb(rgbR + 3 * (roff + x));
+            rowAL[x] = pack_rgb(rgbL + 3 * (roff + x));
+        }
     }
     for (int i = threadIdx.x; i < 65; i += blockDim.x) tC[i] = CENSUS ? tabC[i] : 0.0f;
     for (int i = threadIdx.x; i < nA; i += blockDim.x) tA[i] = tabA[i];
@@ -71,12 +80,11 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
                     xk = xk >= W ? xk - W : xk;
                     const int x = min(x0 + j, W - 1);
                     if (CENSUS && AD)
-                        c[k] = tC[__popcll(cL[roff + x] ^ rowC[xk])] +
-                               tA[sad3(pack_rgb(rgbL + 3 * (roff + x)), rowA[xk])];
+                        c[k] = tC[__popcll(rowCL[x] ^ rowC[xk])] + tA[sad3(rowAL[x], rowA[xk])];
                     else if (CENSUS)
-                        c[k] = tC[__popcll(cL[roff + x] ^ rowC[xk])];
+                        c[k] = tC[__popcll(rowCL[x] ^ rowC[xk])];
                     else
-                        c[k] = tA[sad3(pack_rgb(rgbL + 3 * (roff + x)), rowA[xk])];
+                        c[k] = tA[sad3(rowAL[x], rowA[xk])];
                 }
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
@@ -411,9 +419,20 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
                 if (int e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, s)) return e;
         RTS_MARK(p, "prep", s);
         const int waves = div_up(D, 64);
-        const size_t shm = (size_t)W * ((census ? 8 : 0) + (ad ? 4 : 0)) + sizeof(float) * (65 + p->nA) +
+        const size_t shm = (size_t)W * ((census ? 16 : 0) + (ad ? 8 : 0)) + sizeof(float) * (65 + p->nA) +
                            sizeof(float) * waves * 64 * (FH_CH + 1);
         RTS_CHECK(shm <= 160 * 1024, "front-end LDS %zu too large", shm);
+        if (shm > 65536) {   // the kernel's dynamic LDS above the default 64 KiB
+            static bool raised[3] = {false, false, false};
+            const int which = census && ad ? 0 : census ? 1 : 2;
+            if (!raised[which]) {
+                const void *fn = census && ad ? reinterpret_cast<const void *>(&k_front_hscan<true, true>)
+                                 : census     ? reinterpret_cast<const void *>(&k_front_hscan<true, false>)
+                                              : reinterpret_cast<const void *>(&k_front_hscan<false, true>);
+                RTS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                raised[which] = true;
+            }
+        }
         const int vec = (W % 4) == 0 && ((uintptr_t)cur % 16) == 0;
         if (census && ad)
             k_front_hscan<true, true><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
