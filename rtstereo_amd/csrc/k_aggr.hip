@@ -496,9 +496,11 @@ __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P
                 const float *Pr = Pb + (size_t)min(row, rhi - 1) * W;
                 sv = Pr[min(xc + rr, W - 1)] - Pr[max(xc - l - 1, 0)];
             }
-            if (row < rhi)
-                ring[row & (SW_RING - 1)][lane] = make_float2(
+            if (row < rhi) {
+                const float2 v = make_float2(
                     sv, __uint_as_float((uint32_t)(l + rr + 1) | ((a >> 16) & 0xFFu) << 10 | (a >> 24) << 18));
+                ring[row & (SW_RING - 1)][lane] = v;
+            }
         }
         __syncthreads();
         const int rnext = r + SW_CH;
@@ -513,8 +515,24 @@ __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P
                 float sum = 0.0f;
                 int cnt = 0;
                 if (t <= CV_LMAX && b <= CV_LMAX) {
-                    for (int i = y - t; i <= y + b; i++) {
-                        const float2 e = ring[i & (SW_RING - 1)][lane];
+                    // rows y-t .. y+b, top to bottom (the reference's order),
+                    // four reads issued together
+                    const int n = t + b + 1, y0r = y - t;
+                    int i = 0;
+                    for (; i + 4 <= n; i += 4) {
+                        const float2 e0 = ring[(y0r + i) & (SW_RING - 1)][lane];
+                        const float2 e1 = ring[(y0r + i + 1) & (SW_RING - 1)][lane];
+                        const float2 e2 = ring[(y0r + i + 2) & (SW_RING - 1)][lane];
+                        const float2 e3 = ring[(y0r + i + 3) & (SW_RING - 1)][lane];
+                        sum += e0.x;
+                        sum += e1.x;
+                        sum += e2.x;
+                        sum += e3.x;
+                        cnt += (int)((__float_as_uint(e0.y) & 0x3FF) + (__float_as_uint(e1.y) & 0x3FF) +
+                                     (__float_as_uint(e2.y) & 0x3FF) + (__float_as_uint(e3.y) & 0x3FF));
+                    }
+                    for (; i < n; i++) {
+                        const float2 e = ring[(y0r + i) & (SW_RING - 1)][lane];
                         sum += e.x;
                         cnt += __float_as_uint(e.y) & 0x3FF;
                     }
