@@ -89,6 +89,35 @@ def cpu_baseline(target_s: float) -> dict:
             "frame_seconds_extrapolated": W * H * D / (pd / t)}
 
 
+def aw_timing(stream) -> dict:
+    """Adaptive-weight pipeline (truncated AD 40, 33x33 window, gc 7, gp 36,
+    WTA + check) at 1920x1080, D=192, FP32 and FP16: mean of 3 frames after
+    one warm-up, Lab derived on the device."""
+    from rtstereo_amd import gen, pipeline as PL, _lib as L
+    from rtstereo_amd.device import DeviceArray, Event
+    p = gen.aw_1080p()
+    a, b = DeviceArray.from_host(p.left), DeviceArray.from_host(p.right)
+    res = {"workload": f"AD trunc 40 + adaptive weights 33x33 (gc 7, gp 36) + WTA/check, {W}x{H}, D={D}"}
+    for name, prec in (("fp32", L.RTS_AW_FP32), ("fp16", L.RTS_AW_FP16)):
+        pipe = PL.Pipeline(PL.preset("aw", W, H, D, aw_precision=prec))
+        pipe.enable_timing(True)
+        pipe.run(a, b, stream=stream.handle)
+        e0, e1 = Event(), Event()
+        e0.record(stream.handle)
+        agg = []
+        for _ in range(3):
+            pipe.run(a, b, stream=stream.handle)
+            agg.append(pipe.stage_ms()["aggr"])
+        e1.record(stream.handle)
+        stream.sync()
+        frame = e0.elapsed_ms(e1) / 3
+        res[name] = {"frame_ms": frame, "aggregation_ms": float(np.mean(agg)), "pd_per_s": W * H * D / (frame / 1e3)}
+        pipe.close()
+    a.free()
+    b.free()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +125,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-copy-probe", action="store_true")
+    ap.add_argument("--no-aw", action="store_true", help="skip the adaptive-weight FP32/FP16 timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has one GPU, "
@@ -195,6 +225,8 @@ def main() -> int:
                      "kernel_ms": stage[dom]},
         "copy_kernel_gbs": copy,
     }
+    if not args.no_aw:
+        out["aw"] = aw_timing(stream)
     if args.cpu_seconds > 0:
         cb = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb

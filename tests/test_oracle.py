@@ -395,3 +395,31 @@ def test_median3_brute_force():
             for c in range(3):
                 v = [a[min(max(y + dy, 0), 6), min(max(x + dx, 0), 8), c] for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
                 assert sorted(v)[4] == m[y, x, c]
+
+
+def test_combine3_pins():
+    """The three-term combination: with ncc = 1 the third term vanishes and
+    it is exactly the two-term combine; otherwise it matches the float64
+    formula."""
+    rng = np.random.default_rng(4)
+    c1 = rng.integers(0, 49, (5, 7, 6)).astype(np.float32)
+    c2 = (rng.random((5, 7, 6)) * 40).astype(np.float32)
+    ncc = (rng.random((5, 7, 6)) * 2 - 1).astype(np.float32)
+    assert np.array_equal(oracle.combine3(c1, c2, np.ones_like(ncc), 8.0, 10.0, 0.5), oracle.combine(c1, c2, 8.0, 10.0))
+    got = oracle.combine3(c1, c2, ncc, 8.0, 10.0, 0.5).astype(np.float64)
+    f64 = (1 - np.exp(-c1 / 8.0)) + (1 - np.exp(-c2 / 10.0)) + (1 - np.exp(-(1 - ncc.astype(np.float64)) / 0.5))
+    assert np.max(np.abs(got - f64)) < 1e-6
+
+
+def test_aw_rows_f64_equals_pixel_form():
+    """The row-wise FP64 aggregation (per-row target weight cache, OpenMP)
+    gives the per-pixel FP64 oracle's values."""
+    from rtstereo_amd import gen
+    p = gen.make_pair(48, 32, 12, 4, 2.5, 3)
+    ll, lr = p.lab()
+    c = oracle.ad_color(p.left, p.right, 12, 1, 40.0, False)
+    rows = [0, 7, 31]
+    a = oracle.aw_rows_f64(ll, lr, c, rows, 4, 7.0, 36.0)
+    ys, xs = np.meshgrid(rows, np.arange(48), indexing="ij")
+    b = oracle.aw_l2r_f64(ll, lr, c, ys.ravel(), xs.ravel(), 4, 7.0, 36.0).reshape(3, 48, 12)
+    assert np.array_equal(a, b)
