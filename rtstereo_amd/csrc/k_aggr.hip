@@ -161,6 +161,42 @@ __global__ void __launch_bounds__(256) k_find_cross(const uint8_t *__restrict__ 
     arms[(size_t)y * W + x] = l | (r << 8) | (t << 16) | (b << 24);
 }
 
+// The same for arms up to 17 with the block's neighbourhood (4 + 2*17 rows
+// by 64 + 2*17 columns of packed pixels) staged in LDS.
+constexpr int FC_L = 17;
+__global__ void __launch_bounds__(256) k_find_cross_tile(const uint8_t *__restrict__ img, uint32_t *__restrict__ arms,
+                                                         int W, int H, int L, int tao) {
+    __shared__ uint32_t tile[4 + 2 * FC_L][64 + 2 * FC_L + 1];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
+    constexpr int TH = 4 + 2 * FC_L, TW = 64 + 2 * FC_L;
+    for (int i = threadIdx.y * 64 + threadIdx.x; i < TH * TW; i += 256) {
+        const int r = i / TW, c = i - r * TW;
+        const int yy = y0 - FC_L + r, xx = x0 - FC_L + c;
+        tile[r][c] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? pack_rgb(img + 3 * ((size_t)yy * W + xx)) : 0u;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int tx = threadIdx.x + FC_L, ty = threadIdx.y + FC_L;
+    const uint32_t c = tile[ty][tx];
+    auto len = [&](int sy, int sx) {
+        int n = 0, delta = 0;
+        while (delta < tao && n < L) {
+            const int yy = y + sy * (n + 1), xx = x + sx * (n + 1);
+            if (yy < 0 || yy >= H || xx < 0 || xx >= W) break;
+            n++;
+            delta = max_chan_delta(c, tile[ty + sy * n][tx + sx * n]);
+        }
+        if (n == 0) {
+            const int yy = y + sy, xx = x + sx;
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) n = 1;
+        }
+        return (uint32_t)n;
+    };
+    const uint32_t l = len(0, -1), r = len(0, 1), t = len(-1, 0), b = len(1, 0);
+    arms[(size_t)y * W + x] = l | (r << 8) | (t << 16) | (b << 24);
+}
+
 __global__ void k_arms_unpack(const uint32_t *__restrict__ arms, int *__restrict__ horz, int *__restrict__ vert, int P) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -632,7 +668,10 @@ rts_status rts_find_cross(const uint8_t *rgb, uint32_t *arms, int W, int H, int 
     RTS_CHECK(W >= 1 && H >= 1, "bad image size %dx%d", W, H);
     RTS_CHECK(L >= 0 && L <= 255, "arm limit %d must be in [0, 255]", L);
     dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
-    k_find_cross<<<grid, block, 0, S(stream)>>>(rgb, arms, W, H, L, tao);
+    if (L <= FC_L)
+        k_find_cross_tile<<<grid, block, 0, S(stream)>>>(rgb, arms, W, H, L, tao);
+    else
+        k_find_cross<<<grid, block, 0, S(stream)>>>(rgb, arms, W, H, L, tao);
     RTS_LAUNCHED();
     return RTS_OK;
 }

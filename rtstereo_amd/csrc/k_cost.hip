@@ -85,6 +85,38 @@ __global__ void __launch_bounds__(256) k_census(const uint8_t *__restrict__ img,
     if (k) words[w * pl + (size_t)y * W + x] = acc;
 }
 
+// The same with the block's image rows staged in LDS (radii up to 8).
+constexpr int CT_R = 8;
+__global__ void __launch_bounds__(256) k_census_tile(const uint8_t *__restrict__ img, uint64_t *__restrict__ words,
+                                                     int W, int H, int hr, int wr) {
+    __shared__ uint8_t tile[4 + 2 * CT_R][64 + 2 * CT_R];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
+    const int th = 4 + 2 * hr, tw = 64 + 2 * wr;
+    for (int i = threadIdx.y * 64 + threadIdx.x; i < th * tw; i += 256) {
+        const int r = i / tw, c = i - r * tw;
+        tile[r][c] = img[(size_t)wrapn(y0 - hr + r, H) * W + wrapn(x0 - wr + c, W)];
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+    if (x >= W || y >= H) return;
+    const int tx = threadIdx.x + wr, ty = threadIdx.y + hr;
+    const uint8_t c = tile[ty][tx];
+    const size_t pl = plane(W, H);
+    uint64_t acc = 0;
+    int k = 0, w = 0;
+    for (int dy = -hr; dy <= hr; dy++)
+        for (int dx = -wr; dx <= wr; dx++) {
+            acc |= (uint64_t)(c > tile[ty + dy][tx + dx]) << k;
+            if (++k == 64) {
+                words[w * pl + (size_t)y * W + x] = acc;
+                acc = 0;
+                k = 0;
+                w++;
+            }
+        }
+    if (k) words[w * pl + (size_t)y * W + x] = acc;
+}
+
 __global__ void k_census_unpack(const uint64_t *__restrict__ words, uint8_t *__restrict__ bools, int P, int N) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -513,7 +545,10 @@ rts_status rts_census(const uint8_t *img, uint64_t *words, int W, int H, int hr,
     RTS_CHECK(W >= 1 && H >= 1, "bad image size %dx%d", W, H);
     RTS_CHECK(hr >= 0 && wr >= 0 && hr < H && wr < W, "census radii %d,%d do not fit the image", hr, wr);
     dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
-    k_census<<<grid, block, 0, S(stream)>>>(img, words, W, H, hr, wr);
+    if (hr <= CT_R && wr <= CT_R)
+        k_census_tile<<<grid, block, 0, S(stream)>>>(img, words, W, H, hr, wr);
+    else
+        k_census<<<grid, block, 0, S(stream)>>>(img, words, W, H, hr, wr);
     RTS_LAUNCHED();
     return RTS_OK;
 }
