@@ -214,8 +214,10 @@ rts_status rts_hprefix(float *vol, int W, int H, int D, void *stream);
  * sums P[x+r] - P[max(x-l-1,0)] added in the reference's row order, then
  * divided by the count -- bit-identical to
  * seq/CrossBasedAggregation.cpp:127-169 cross_stereo_aggregation.  Both mode
- * values run the same kernel (RTS_CROSS_FAST is kept as an alias). */
-enum { RTS_CROSS_EXACT = 0, RTS_CROSS_FAST = 1 };
+ * values run the same kernel (RTS_CROSS_FAST is kept as an alias).  OR-ing
+ * RTS_CROSS_ARMS_LE17 into mode promises that no arm exceeds 17 (the
+ * reference's L), which selects a sweep without the long-arm fallbacks. */
+enum { RTS_CROSS_EXACT = 0, RTS_CROSS_FAST = 1, RTS_CROSS_ARMS_LE17 = 0x100 };
 rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W, int H,
                        int D, int mode, void *stream);
 

@@ -25,7 +25,7 @@ RTS_OK, RTS_ERR_INVALID, RTS_ERR_HIP, RTS_ERR_NOMEM, RTS_ERR_BUDGET, RTS_ERR_CAN
 RTS_L2R, RTS_R2L = 0, 1
 RTS_NCC_SEQ, RTS_NCC_INTEGRAL, RTS_NCC_KEEP_FLAT = 0, 1, 0x100
 RTS_AW_FP32, RTS_AW_FP16 = 0, 1
-RTS_CROSS_EXACT, RTS_CROSS_FAST = 0, 1
+RTS_CROSS_EXACT, RTS_CROSS_FAST, RTS_CROSS_ARMS_LE17 = 0, 1, 0x100
 RTS_WTA_MAX, RTS_WTA_LITERAL = 1, 2
 RTS_COST_AD, RTS_COST_RANK, RTS_COST_CENSUS, RTS_COST_LOG, RTS_COST_NCC = 1, 2, 3, 4, 5
 RTS_AGGR_NONE, RTS_AGGR_BOX, RTS_AGGR_AW, RTS_AGGR_CROSS = 0, 1, 2, 3

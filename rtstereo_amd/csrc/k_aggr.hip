@@ -476,6 +476,9 @@ __device__ __forceinline__ uint32_t min_u8x4(uint32_t a, uint32_t b) {
     return lo | (hi << 8);
 }
 
+// LONG: arms may exceed CV_LMAX (cross_L > 17), with fallbacks that read
+// the prefix volume and the arm maps directly.
+template <bool LONG>
 __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P, float *__restrict__ out,
                                                      const uint32_t *__restrict__ aref,
                                                      const uint32_t *__restrict__ atar, int W, int H, int nx,
@@ -524,7 +527,7 @@ __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P
             const uint32_t a = min_u8x4(ar[k], at[k]);
             const int l = arm_l(a), rr = arm_r(a);
             float sv;
-            if (l <= CV_LMAX && rr <= CV_LMAX) {
+            if (!LONG || (l <= CV_LMAX && rr <= CV_LMAX)) {
                 const int xl = max(xc - l - 1, 0), xr = min(xc + rr, W - 1);
                 sv = lane_fetch(p0[k], p1[k], xr - cs) - lane_fetch(p0[k], p1[k], xl - cs);
             } else {
@@ -550,7 +553,7 @@ __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P
                 const int t = (word >> 10) & 0xFF, b = word >> 18;
                 float sum = 0.0f;
                 int cnt = 0;
-                if (t <= CV_LMAX && b <= CV_LMAX) {
+                if (!LONG || (t <= CV_LMAX && b <= CV_LMAX)) {
                     // rows y-t .. y+b, top to bottom (the reference's order),
                     // four reads issued together
                     const int n = t + b + 1, y0r = y - t;
@@ -596,16 +599,10 @@ using namespace rts;
 template <bool DIRECT>
 static int launch_cross_tile(const float *in, float *out, const uint32_t *ar, const uint32_t *at, int W, int H, int D,
                              hipStream_t s) {
-    static const int ty = [] {
-        const char *e = getenv("RTS_CROSS_TY");
-        return (e && atoi(e) == 64) ? 64 : 32;
-    }();
+    constexpr int ty = 32;
     RTS_CHECK(div_up(H, ty) <= 65535, "height %d too large", H);
     dim3 grid(div_up(W, CV_TX), div_up(H, ty), D);
-    if (ty == 64)
-        k_cross_tile<DIRECT, 64><<<grid, 256, 0, s>>>(in, out, ar, at, W, H);
-    else
-        k_cross_tile<DIRECT, 32><<<grid, 256, 0, s>>>(in, out, ar, at, W, H);
+    k_cross_tile<DIRECT, ty><<<grid, 256, 0, s>>>(in, out, ar, at, W, H);
     RTS_LAUNCHED();
     return RTS_OK;
 }
@@ -699,18 +696,19 @@ rts_status rts_hprefix(float *vol, int W, int H, int D, void *stream) {
     return RTS_OK;
 }
 
-// Largest arm in either map bounds the tile halo; arms above CV_LMAX are
-// rejected (the reference's arms are at most 17).
-
 rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W, int H,
                        int D, int mode, void *stream) {
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(P != out, "cross aggregation is out of place");
-    RTS_CHECK(mode == RTS_CROSS_EXACT || mode == RTS_CROSS_FAST, "bad cross mode %d", mode);
+    const int base_mode = mode & ~RTS_CROSS_ARMS_LE17;
+    RTS_CHECK(base_mode == RTS_CROSS_EXACT || base_mode == RTS_CROSS_FAST, "bad cross mode %d", mode);
     const int nx = div_up(W, CV_TX), nseg = div_up(H, SW_SEG);
     RTS_CHECK((long)nx * nseg * D < (1L << 31), "volume too large");
     // Both mode values run the same sweep, which adds in the reference's order.
-    k_cross_strip<<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
+    if (mode & RTS_CROSS_ARMS_LE17)
+        k_cross_strip<false><<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
+    else
+        k_cross_strip<true><<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
     RTS_LAUNCHED();
     return RTS_OK;
 }

@@ -482,7 +482,8 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
                 // front end for the first pass), then the vertical sweep
                 if (!prefixed)
                     if (int e = rts_hprefix(cur, W, H, D, s)) return e;
-                const int mode = (d.cross_exact || d.cross_L > 17) ? RTS_CROSS_EXACT : RTS_CROSS_FAST;
+                const int mode = ((d.cross_exact || d.cross_L > 17) ? RTS_CROSS_EXACT : RTS_CROSS_FAST) |
+                                 (d.cross_L <= 17 ? RTS_CROSS_ARMS_LE17 : 0);
                 if (int e = rts_cross_v(cur, other, p->arms[0], p->arms[1], W, H, D, mode, s)) return e;
                 prefixed = false;
                 std::swap(cur, other);

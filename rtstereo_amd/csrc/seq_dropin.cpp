@@ -15,6 +15,8 @@
 #include "rtstereo.h"
 #include "rtstereo_seq.h"
 
+#include <algorithm>
+
 namespace {
 
 thread_local std::string g_seq_err;
@@ -429,7 +431,11 @@ void cross_stereo_aggregation(float *cost_image, float *cost_new_image, int *Hor
     STEP(rts_stream_sync(k.s()));
     STEP(arms_up(k, HorzOffset_tar, VertOffset_tar, at, cdim, rdim, 6));
     STEP(vol_up(k, cost_image, a, cdim, rdim, dispRange));
-    STEP(rts_cross_v(a, b, ar, at, cdim, rdim, dispRange, RTS_CROSS_EXACT, k.s()));
+    int longest = 0;   // arms of at most 17 select the sweep without long-arm fallbacks
+    for (const int *m : {HorzOffset_ref, HorzOffset_tar, VertOffset_ref, VertOffset_tar})
+        for (size_t i = 0; i < 2 * P(rdim, cdim); i++) longest = std::max(longest, m[i]);
+    STEP(rts_cross_v(a, b, ar, at, cdim, rdim, dispRange, RTS_CROSS_EXACT | (longest <= 17 ? RTS_CROSS_ARMS_LE17 : 0),
+                     k.s()));
     STEP(vol_down(k, b, cost_new_image, cdim, rdim, dispRange));
 }
 

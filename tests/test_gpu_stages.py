@@ -186,6 +186,28 @@ def test_aw_bands_full_width():
             assert np.max(np.abs(got - f64)) < 0.05
 
 
+@pytest.mark.parametrize("Lm,tao", [(40, 30), (60, 255)])
+def test_cross_long_arms(Lm, tao):
+    """Arms longer than 17 (beyond the sweep's ring window) take the sweep's
+    fallback paths; the short-arm flag must not be used for them, and with
+    arms of at most 17 it gives the same values."""
+    W, H, D = 130, 70, 24
+    p = _pair(W, H, D, 21)
+    gl, gr = p.gray()
+    c = oracle.combine(oracle.hamming(oracle.census(gl, 4, 3), oracle.census(gr, 4, 3), 4, 3, D),
+                       oracle.ad_color(p.left, p.right, D, 1), 30.0, 10.0)
+    P = oracle.hprefix(c)
+    al, hl, vl = S.find_cross(p.left, Lm, tao)
+    ar, hr, vr = S.find_cross(p.right, Lm, tao)
+    assert max(hl.max(), vl.max(), hr.max(), vr.max()) > 17
+    A = oracle.cross_v(P, hl, hr, vl, vr)
+    assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT)), A)
+    al, hl, vl = S.find_cross(p.left, 17, tao)
+    ar, hr, vr = S.find_cross(p.right, 17, tao)
+    A = oracle.cross_v(P, hl, hr, vl, vr)
+    assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT | L.RTS_CROSS_ARMS_LE17)), A)
+
+
 @pytest.mark.parametrize("W,H,D,seed", SIZES)
 def test_cross(W, H, D, seed):
     p = _pair(W, H, D, seed)
