@@ -36,7 +36,7 @@ PEAK_HBM_GBS = 8000.0
 # the fused cost + row-prefix kernel writes the prefix volume once; the
 # vertical sweep reads it and writes the aggregated volume; WTA reads that.
 STAGE_BYTES_PER_PD = {"cost": 4.0, "aggr": 8.0, "wta": 4.0}
-STAGE_KERNEL = {"cost": "k_front_hscan", "aggr": "k_cross_strip", "wta": "k_wta"}
+STAGE_KERNEL = {"cost": "k_front_hscan", "aggr": "k_cross_sweep", "wta": "k_wta"}
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
 # the copy kernel's reading of the same counters (FETCH x2.000, WRITE x1.000);
 # profiles/census_cross_1080p_d192.md holds the runs.

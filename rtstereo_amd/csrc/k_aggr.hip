@@ -476,9 +476,9 @@ __device__ __forceinline__ uint32_t min_u8x4(uint32_t a, uint32_t b) {
     return lo | (hi << 8);
 }
 
-// LONG: arms may exceed CV_LMAX (cross_L > 17), with fallbacks that read
-// the prefix volume and the arm maps directly.
-template <bool LONG>
+// This form serves arms longer than CV_LMAX (cross_L > 17): a segment or a
+// vertical support longer than the staged window falls back to reading the
+// prefix volume and the arm maps directly.
 __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P, float *__restrict__ out,
                                                      const uint32_t *__restrict__ aref,
                                                      const uint32_t *__restrict__ atar, int W, int H, int nx,
@@ -527,7 +527,7 @@ __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P
             const uint32_t a = min_u8x4(ar[k], at[k]);
             const int l = arm_l(a), rr = arm_r(a);
             float sv;
-            if (!LONG || (l <= CV_LMAX && rr <= CV_LMAX)) {
+            if (l <= CV_LMAX && rr <= CV_LMAX) {
                 const int xl = max(xc - l - 1, 0), xr = min(xc + rr, W - 1);
                 sv = lane_fetch(p0[k], p1[k], xr - cs) - lane_fetch(p0[k], p1[k], xl - cs);
             } else {
@@ -553,7 +553,7 @@ __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P
                 const int t = (word >> 10) & 0xFF, b = word >> 18;
                 float sum = 0.0f;
                 int cnt = 0;
-                if (!LONG || (t <= CV_LMAX && b <= CV_LMAX)) {
+                if (t <= CV_LMAX && b <= CV_LMAX) {
                     // rows y-t .. y+b, top to bottom (the reference's order),
                     // four reads issued together
                     const int n = t + b + 1, y0r = y - t;
@@ -592,9 +592,196 @@ __global__ void __launch_bounds__(256) k_cross_strip(const float *__restrict__ P
     }
 }
 
+// Element at a 32-bit byte offset from a uniform base, so that the load can
+// take the scalar-base form with a 32-bit per-lane offset.
+template <typename T>
+__device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
+    return *(const T *)((const char *)base + off);
+}
+
+// a / c for a support count c in [1, 2^19], rounded as IEEE division rounds
+// it.  With y = rcp(c) (relative error e, |e| <= 2^-22), q0 = a*y is within a
+// few ulp of a/c; the residual r0 = a - c*q0 is then exact (a multiple of
+// ulp(q0) no larger than 2^24 of them, since c is an integer), and
+// q0 + r0*y = a/c + (a/c - q0)*e lies within 2^-19 ulp of a/c.  A quotient a/c
+// of a float by an integer c is never a rounding midpoint in the normal range
+// and lies at least ulp/(2c) from one, so the fused q1 = RN(q0 + r0*y) is
+// RN(a/c).  Results that come out denormal, infinite or NaN (where midpoints
+// do occur, or the residual is not finite) are recomputed by the division.
+__device__ __forceinline__ float div_count(float a, uint32_t c) {
+    const float cf = (float)c;
+    const float y = __builtin_amdgcn_rcpf(cf);
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-q0, cf, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    // classes: signalling/quiet NaN, -inf, -denormal, +denormal, +inf
+    if (__builtin_expect(__builtin_amdgcn_classf(q1, 0x297), 0)) return __fdiv_rn(a, cf);
+    return q1;
+}
+
+// The sweep for arms of at most CV_LMAX (seq/CrossBasedAggregation.cpp:127-169,
+// bit-identical).  A block of NW waves owns 64 columns and SEG output rows of
+// one d plane and walks rows [s0-17, s0+SEG+17) in steps of CH; every wave
+// forms CH/NW rows of segment sums per step and then emits every output row
+// whose support is complete, NW rows apart.
+//  - Segment sums: a row's 99 prefix values (columns x0-18 .. x0+80) are
+//    loaded coalesced, two registers per lane, one step ahead; each lane
+//    picks its segment's two ends with ds_bpermute, so every segment sum is
+//    formed once (no tile halo).
+//  - The ring keeps R = CH + 34 live rows and repeats its first 34 rows past
+//    its end, so the rows y-17 .. y+17 of any output sit at consecutive
+//    positions and every read of a vertical sum is an immediate offset from
+//    one per-lane base (two rows per ds_read2st64_b64).
+//  - A ring entry is (segment sum, len | t << 16 | b << 24): the low 16 bits
+//    of the plain 32-bit sum of the entries over a support are its count (at
+//    most 35 rows of at most 35 columns), so no field is unpacked per row.
+//  - A support of n rows is summed as n % 4 single rows and then n / 4
+//    groups of four, in row order from +0.0 (the reference's order), with
+//    exec-masked guards instead of loops; the quotient uses div_count.
+// Eight waves share one ring, which keeps six waves per SIMD resident at
+// 51 KB of LDS per block.
+template <int CH, int NW>
+__global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict__ P, float *__restrict__ out,
+                                                         const uint32_t *__restrict__ aref,
+                                                         const uint32_t *__restrict__ atar, int W, int H, int nx,
+                                                         int nseg, int seg, int D) {
+    constexpr int R = CH + 2 * CV_LMAX;        // rows live at once
+    constexpr int RR = R + 2 * CV_LMAX;        // stored rows: the first 34 repeated past the end
+    constexpr int KR = CH / NW;                // rows per wave per step
+    static_assert(KR * NW == CH, "step rows must split evenly over the waves");
+    __shared__ float2 ring[RR][CV_TX];
+    const int wk = xcd_order(blockIdx.x, nx * nseg * D);
+    const int xt = wk % nx, rest = wk / nx;
+    const int d = rest % D, sg = rest / D;
+    const int x0 = xt * CV_TX, s0 = sg * seg;
+    const int s1 = min(H, s0 + seg);
+    const size_t base = (size_t)d * plane(W, H);
+    const int rlo = max(0, s0 - CV_LMAX), rhi = min(H, s1 + CV_LMAX);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int x = x0 + lane;
+    const bool colok = x < W;
+    const int xc = min(x, W - 1);
+    const int tx = wrap1(xc - d, W);
+    const int cs = x0 - CV_LMAX - 1;
+    const int c0 = cs + lane, c1 = cs + 64 + lane;
+    const bool ok0 = c0 >= 0 && c0 < W, ok1 = lane < CV_COLS - 64 && c1 >= 0 && c1 < W;
+    const float *Pb = P + base;
+    // uniform row bases with 32-bit per-lane byte offsets (scalar-base addressing)
+    const uint32_t oxc = (uint32_t)xc * 4u, otx = (uint32_t)tx * 4u;
+    const uint32_t oc0 = (uint32_t)c0 * 4u, oc1 = (uint32_t)c1 * 4u;
+    // Per row and lane: the 99 prefix values of the row's segments held as
+    // two registers (columns cs+lane and cs+64+lane), and the two arm words.
+    // A step's loads are issued one step before its segment sums are formed.
+    float p0[KR], p1[KR];
+    uint32_t ar[KR], at[KR];
+    auto load_rows = [&](int r) {
+#pragma unroll
+        for (int k = 0; k < KR; k++) {
+            const size_t ro = (size_t)min(r + w + NW * k, rhi - 1) * W;
+            p0[k] = ok0 ? at_byte(Pb + ro, oc0) : 0.0f;
+            p1[k] = ok1 ? at_byte(Pb + ro, oc1) : 0.0f;
+            ar[k] = at_byte(aref + ro, oxc);
+            at[k] = at_byte(atar + ro, otx);
+        }
+    };
+    int out_next = s0;
+    int pr = 0;   // ring position of row r: (r - rlo) mod R
+    load_rows(rlo);
+    for (int r = rlo; r < rhi; r += CH) {
+#pragma unroll
+        for (int k = 0; k < KR; k++) {
+            const uint32_t a = ar[k], c = at[k];
+            const uint32_t l = min(a & 0xFFu, c & 0xFFu), rr = min((a >> 8) & 0xFFu, (c >> 8) & 0xFFu);
+            const uint32_t t = min((a >> 16) & 0xFFu, (c >> 16) & 0xFFu), b = min(a >> 24, c >> 24);
+            // columns of the segment's ends relative to cs (ds_bpermute takes
+            // the source lane from address bits 7:2, so j*4 selects lane j % 64)
+            const int jl = max(xc - (int)l - 1, 0) - cs, jr = min(xc + (int)rr, W - 1) - cs;
+            const int bl = __builtin_amdgcn_ds_bpermute(jl * 4, __float_as_int(p0[k]));
+            const int bh = __builtin_amdgcn_ds_bpermute(jl * 4, __float_as_int(p1[k]));
+            const int el = __builtin_amdgcn_ds_bpermute(jr * 4, __float_as_int(p0[k]));
+            const int eh = __builtin_amdgcn_ds_bpermute(jr * 4, __float_as_int(p1[k]));
+            const float sv = __int_as_float(jr < 64 ? el : eh) - __int_as_float(jl < 64 ? bl : bh);
+            if (r + w + NW * k < rhi) {
+                const float2 v = make_float2(sv, __uint_as_float((l + rr + 1) | t << 16 | b << 24));
+                int p = pr + w + NW * k;
+                if (p >= R) p -= R;
+                ring[p][lane] = v;
+                if (p < 2 * CV_LMAX) ring[p + R][lane] = v;
+            }
+        }
+        __syncthreads();
+        const int rnext = r + CH;
+        if (rnext < rhi) load_rows(rnext);
+        const int avail = min(rnext, rhi);
+        const int out_end = avail >= rhi ? s1 : min(s1, avail - CV_LMAX);
+        if (colok) {
+            // ring position of row y-17 for the wave's first output row (rows
+            // below rlo never enter a support), then advanced with y
+            int bm = (out_next + w - CV_LMAX - rlo + R) % R;
+            for (int y = out_next + w; y < out_end; y += NW) {
+                const float2 *col = &ring[bm][lane];
+                bm += NW;
+                if (bm >= R) bm -= R;
+                const uint32_t word = __float_as_uint(col[CV_LMAX * CV_TX].y);
+                const int t = (word >> 16) & 0xFF, b = word >> 24;
+                const int n = t + b + 1, lead = n & 3, q = n >> 2;
+                const float2 *s = col + (CV_LMAX - t) * CV_TX;
+                float sum = 0.0f;
+                uint32_t cw = 0;
+                if (lead) {
+                    float2 e = s[0];
+                    sum += e.x;
+                    cw += __float_as_uint(e.y);
+                    if (lead > 1) {
+                        e = s[CV_TX];
+                        sum += e.x;
+                        cw += __float_as_uint(e.y);
+                        if (lead > 2) {
+                            e = s[2 * CV_TX];
+                            sum += e.x;
+                            cw += __float_as_uint(e.y);
+                        }
+                    }
+                }
+                s += lead * CV_TX;
+#pragma unroll
+                for (int gi = 0; gi < (2 * CV_LMAX + 1) / 4; gi++) {
+                    if (gi >= q) break;
+                    const float2 e0 = s[(4 * gi) * CV_TX], e1 = s[(4 * gi + 1) * CV_TX];
+                    const float2 e2 = s[(4 * gi + 2) * CV_TX], e3 = s[(4 * gi + 3) * CV_TX];
+                    sum += e0.x;
+                    sum += e1.x;
+                    sum += e2.x;
+                    sum += e3.x;
+                    cw += __float_as_uint(e0.y) + __float_as_uint(e1.y) + __float_as_uint(e2.y) +
+                          __float_as_uint(e3.y);
+                }
+                float *orow = out + base + (size_t)y * W;
+                *(float *)((char *)orow + (uint32_t)x * 4u) = div_count(sum, cw & 0xFFFFu);
+            }
+        }
+        out_next = max(out_next, out_end);
+        pr += CH;
+        if (pr >= R) pr -= R;
+        __syncthreads();
+    }
+}
+
 }  // namespace rts
 
 using namespace rts;
+
+// Shape of the short-arm sweep: 32 rows per step over 8 waves; a block owns
+// a column strip of XS_SEG_MIN.. rows, cut so that the grid holds about 4096
+// blocks (whole columns for 1080p at D >= 72).
+constexpr int XS_CH = 32, XS_NW = 8, XS_SEG_MIN = 64;
+static int xs_segments(int W, int H, int D) {
+    const long strips = (long)div_up(W, CV_TX) * D;
+    const long most = H / XS_SEG_MIN > 1 ? H / XS_SEG_MIN : 1;
+    const long want = (4096L + strips - 1) / strips;
+    return (int)(want < most ? want : most);
+}
 
 template <bool DIRECT>
 static int launch_cross_tile(const float *in, float *out, const uint32_t *ar, const uint32_t *at, int W, int H, int D,
@@ -705,10 +892,14 @@ rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, con
     const int nx = div_up(W, CV_TX), nseg = div_up(H, SW_SEG);
     RTS_CHECK((long)nx * nseg * D < (1L << 31), "volume too large");
     // Both mode values run the same sweep, which adds in the reference's order.
-    if (mode & RTS_CROSS_ARMS_LE17)
-        k_cross_strip<false><<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
-    else
-        k_cross_strip<true><<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
+    if (mode & RTS_CROSS_ARMS_LE17) {
+        const int nsg = xs_segments(W, H, D), seg = div_up(H, nsg);
+        RTS_CHECK((long)nx * nsg * D < (1L << 31), "volume too large");
+        k_cross_sweep<XS_CH, XS_NW><<<nx * nsg * D, 64 * XS_NW, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx,
+                                                                               nsg, seg, D);
+    } else {
+        k_cross_strip<<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
+    }
     RTS_LAUNCHED();
     return RTS_OK;
 }

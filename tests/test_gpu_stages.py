@@ -225,6 +225,7 @@ def test_cross(W, H, D, seed):
     ar, hr, vr = S.find_cross(p.right)
     A = oracle.cross_v(P, hl, hr, vl, vr)
     assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT)), A)
+    assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT | L.RTS_CROSS_ARMS_LE17)), A)
     f64 = oracle.cross_f64(c, hl, hr, vl, vr)
     fastv = pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_FAST)).astype(np.float64)
     assert np.max(np.abs(fastv - A) / np.maximum(A, 1e-30)) <= 1e-5
@@ -235,6 +236,46 @@ def test_cross(W, H, D, seed):
     Ai = oracle.cross_v(oracle.hprefix(ci), hl, hr, vl, vr)
     assert np.array_equal(pm(S.cross_v(dm(oracle.hprefix(ci)), al, ar, L.RTS_CROSS_FAST)), Ai)
     assert np.array_equal(pm(S.cross_fused(dm(ci), al, ar)), Ai)
+
+
+@pytest.mark.parametrize("W,H,D,seed", [(150, 300, 16, 31), (100, 1000, 4, 32)])
+def test_cross_short_arm_segments(W, H, D, seed):
+    """Tall images split each column strip of the short-arm sweep into
+    several row segments (4 of 75 rows, 15 of 67 with a shorter last one);
+    every segment boundary must give the loop form's values bit for bit."""
+    p = _pair(W, H, D, seed)
+    gl, gr = p.gray()
+    c = oracle.combine(oracle.hamming(oracle.census(gl, 4, 3), oracle.census(gr, 4, 3), 4, 3, D),
+                       oracle.ad_color(p.left, p.right, D, 1), 30.0, 10.0)
+    P = oracle.hprefix(c)
+    al, hl, vl = S.find_cross(p.left)
+    ar, hr, vr = S.find_cross(p.right)
+    A = oracle.cross_v(P, hl, hr, vl, vr)
+    assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT | L.RTS_CROSS_ARMS_LE17)), A)
+
+
+def test_cross_short_arm_special_values():
+    """Denormal, infinite and NaN support sums take the quotient's IEEE
+    division path; zeros and ordinary values the reciprocal path.  Both must
+    equal the loop form (NaN positions included)."""
+    W, H, D = 96, 80, 6
+    p = _pair(W, H, D, 33)
+    rng = np.random.default_rng(33)
+    c = rng.random((H, W, D)).astype(np.float32)
+    c[:, :, 1] *= np.float32(1e-39)                 # denormal costs and sums
+    c[:, :, 2] = 0.0                                # exact zeros
+    c[:, :, 3] *= np.float32(3e-38)                 # quotients near the normal limit
+    c[rng.random((H, W)) < 0.002, 4] = np.inf       # inf and inf - inf = NaN segments
+    c[rng.random((H, W)) < 0.002, 5] = np.nan
+    P = oracle.hprefix(c)
+    al, hl, vl = S.find_cross(p.left)
+    ar, hr, vr = S.find_cross(p.right)
+    A = oracle.cross_v(P, hl, hr, vl, vr)
+    got = pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT | L.RTS_CROSS_ARMS_LE17))
+    assert np.isnan(A).any() and (np.abs(A[np.isfinite(A)]) < np.float32(1.2e-38)).any()
+    assert np.array_equal(np.isnan(got), np.isnan(A))
+    fin = ~np.isnan(A)
+    assert np.array_equal(got[fin].view(np.uint32), A[fin].view(np.uint32))
 
 
 @pytest.mark.parametrize("W,H,D,seed", SIZES)
