@@ -1,98 +1,99 @@
 // Disparity selection: winner-takes-all with the right view read off the
-// volume's diagonal and the left-right check fused in, the stand-alone
-// cross-check, and the cross-support histogram refinement.
+// volume's diagonal, the left-right check, and the cross-support histogram
+// refinement.
 #include "rts_internal.h"
 
 namespace rts {
 
-// One block per image row; thread t owns pixels t, t+256, ...  For every d
-// it reads vol[d][y][x] for the left view and vol[d][y][(x+d) mod W] for the
-// right view (right pixel x matched with left pixel x+d); both reads are
-// contiguous across the wavefront, and the second mostly hits the cache
-// lines the first brought in.  Strict comparisons from d=0 up keep the
-// lowest d on ties (seq/WTA.cpp:16-21).  WTA_DU disparities are loaded
-// before they are compared, so their loads are in flight together.  After
-// the sweep the row's D_L and D_R go to LDS and the left-right check
-// (seq/CrossCheck.cpp:15-21) runs there.
-constexpr int WTA_THREADS = 256;
-constexpr int WTA_DU = 4;
+// A block of T threads owns a chunk of T*PPT pixels of one row; thread t
+// owns pixels t, t+T, ...  For every d it reads vol[d][y][x] for the left view and
+// vol[d][y][(x+d) mod W] for the right view (right pixel x matched with left
+// pixel x+d); both reads are contiguous across the wavefront, and the second
+// mostly hits the cache lines the first brought in.  d = 0 seeds both
+// searches and strict comparisons from d = 1 up keep the lowest d on ties
+// (seq/WTA.cpp:16-21).  DU disparities are loaded before they are compared,
+// so their loads are in flight together.  The left-right check
+// (seq/CrossCheck.cpp:15-21) is k_cross_check on the finished maps.
 
-template <int PPT, bool MAXIMIZE>
-__global__ void __launch_bounds__(WTA_THREADS) k_wta(const float *__restrict__ vol, uint8_t *__restrict__ dl,
-                                                     uint8_t *__restrict__ dr, uint8_t *__restrict__ mask,
-                                                     uint8_t *__restrict__ beat0, int W, int H, int D, int tol) {
-    extern __shared__ uint8_t maps[];   // 2*W bytes: D_L then D_R of the row
-    uint8_t *mapL = maps, *mapR = maps + W;
-    const int y = blockIdx.x;
+template <typename T>
+__device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
+    return *(const T *)((const char *)base + off);
+}
+
+// A value the compiler cannot see through, so that a 32-bit offset stays a
+// 32-bit per-lane operand of a scalar-base load.
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+template <bool MAXIMIZE>
+__device__ __forceinline__ bool beats(float v, float best) {
+    return MAXIMIZE ? v > best : v < best;
+}
+
+template <int PPT, bool MAXIMIZE, bool WANT_R, int DU, int T>
+__global__ void __launch_bounds__(T) k_wta(const float *__restrict__ vol, uint8_t *__restrict__ dl,
+                                                     uint8_t *__restrict__ dr, uint8_t *__restrict__ beat0, int W,
+                                                     int H, int D) {
+    const int y = blockIdx.y;
+    const int xb = blockIdx.x * (T * PPT) + threadIdx.x;
     const size_t pl = plane(W, H), rowoff = (size_t)y * W;
-    const bool want_r = dr != nullptr;
     const float *row = vol + rowoff;
+    // byte offsets of the thread's pixels, and of the same pixels minus a row
+    // (the right view reads column (x + d) mod W = min(x+d, x+d-W) unsigned)
+    uint32_t ol[PPT], om[PPT];
     float bestL[PPT], bestR[PPT];
-    int dL[PPT], dR[PPT], xs[PPT];
+    int dL[PPT], dR[PPT];
 #pragma unroll
     for (int i = 0; i < PPT; i++) {
-        xs[i] = min((int)threadIdx.x + i * WTA_THREADS, W - 1);
-        bestL[i] = bestR[i] = 0.0f;
+        const uint32_t x = (uint32_t)min(xb + i * T, W - 1);
+        ol[i] = x * 4u;
+        om[i] = (x - (uint32_t)W) * 4u;
+        bestL[i] = at_byte(row, ol[i]);   // d = 0 seeds the search
+        bestR[i] = bestL[i];
         dL[i] = dR[i] = 0;
     }
-    for (int d0 = 0; d0 < D; d0 += WTA_DU) {
-        float vL[WTA_DU][PPT], vR[WTA_DU][PPT];
+    for (int d0 = 1; d0 < D; d0 += DU) {
+        float vL[DU][PPT], vR[DU][PPT];
 #pragma unroll
-        for (int j = 0; j < WTA_DU; j++) {
+        for (int j = 0; j < DU; j++) {
             const int d = min(d0 + j, D - 1);
             const float *pr = row + (size_t)d * pl;
+            const uint32_t d4 = (uint32_t)d * 4u;
 #pragma unroll
             for (int i = 0; i < PPT; i++) {
-                vL[j][i] = pr[xs[i]];
-                if (want_r) {
-                    int xr = xs[i] + d;
-                    if (xr >= W) xr -= W;
-                    vR[j][i] = pr[xr];
-                }
+                vL[j][i] = at_byte(pr, opaque_u32(ol[i]));
+                if (WANT_R) vR[j][i] = at_byte(pr, min(ol[i] + d4, om[i] + d4));
             }
         }
+        // every load of the batch is issued before the first comparison waits
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int j = 0; j < WTA_DU; j++) {
+        for (int j = 0; j < DU; j++) {
             const int d = d0 + j;
             if (d >= D) break;
 #pragma unroll
             for (int i = 0; i < PPT; i++) {
-                const float v = vL[j][i];
-                if (d == 0 || (MAXIMIZE ? (v > bestL[i]) : (v < bestL[i]))) {
-                    bestL[i] = v;
+                if (beats<MAXIMIZE>(vL[j][i], bestL[i])) {
+                    bestL[i] = vL[j][i];
                     dL[i] = d;
                 }
-                if (want_r) {
-                    const float u = vR[j][i];
-                    if (d == 0 || (MAXIMIZE ? (u > bestR[i]) : (u < bestR[i]))) {
-                        bestR[i] = u;
-                        dR[i] = d;
-                    }
+                if (WANT_R && beats<MAXIMIZE>(vR[j][i], bestR[i])) {
+                    bestR[i] = vR[j][i];
+                    dR[i] = d;
                 }
             }
         }
     }
 #pragma unroll
     for (int i = 0; i < PPT; i++) {
-        const int x = threadIdx.x + i * WTA_THREADS;
+        const int x = xb + i * T;
         if (x < W) {
             dl[rowoff + x] = (uint8_t)dL[i];
             if (beat0) beat0[rowoff + x] = dL[i] != 0;
-            if (want_r) {
-                dr[rowoff + x] = (uint8_t)dR[i];
-                mapL[x] = (uint8_t)dL[i];
-                mapR[x] = (uint8_t)dR[i];
-            }
+            if (WANT_R) dr[rowoff + x] = (uint8_t)dR[i];
         }
-    }
-    if (!mask) return;
-    __syncthreads();
-    for (int x = threadIdx.x; x < W; x += WTA_THREADS) {
-        const int v = mapL[x];
-        int xt = x - v;
-        if (xt < 0) xt += W;
-        const int t = mapR[xt];
-        mask[rowoff + x] = (uint8_t)(abs(v - t) <= tol);
     }
 }
 
@@ -216,31 +217,38 @@ __global__ void k_disp_refine_literal(uint8_t *__restrict__ disp, const uint32_t
         }
 }
 
+// Rows are cut into chunks, one block each.  One wave per block and 8 pixels per lane (512-pixel row chunks): a 1080p
+// frame gives 4320 blocks, all resident at once, spread evenly over the CUs,
+// with 2*4*8 loads in flight per lane.
+constexpr int WTA_T = 64, WTA_DU = 4;
+
+template <int PPT, bool MAXIMIZE, bool WANT_R>
+static void wta_go(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *beat, int W, int H, int D, hipStream_t s) {
+    k_wta<PPT, MAXIMIZE, WANT_R, WTA_DU, WTA_T><<<dim3(div_up(W, WTA_T * PPT), H), WTA_T, 0, s>>>(vol, dl, dr, beat, W,
+                                                                                                 H, D);
+}
+
+template <bool MAXIMIZE, bool WANT_R>
+static void launch_wta_r(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *beat, int W, int H, int D,
+                         hipStream_t s) {
+    if (W <= 64)
+        wta_go<1, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+    else if (W <= 128)
+        wta_go<2, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+    else if (W <= 256)
+        wta_go<4, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+    else
+        wta_go<8, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+}
+
 template <bool MAXIMIZE>
-static int launch_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, uint8_t *beat, int W, int H, int D,
-                      int tol, hipStream_t s) {
-    const int ppt = div_up(W, WTA_THREADS);
-    const size_t shm = 2 * (size_t)W;
-#define RTS_WTA_CASE(N)                                                                         \
-    case N:                                                                                     \
-        k_wta<N, MAXIMIZE><<<H, WTA_THREADS, shm, s>>>(vol, dl, dr, mask, beat, W, H, D, tol); \
-        break;
-    switch (ppt) {
-        RTS_WTA_CASE(1)
-        RTS_WTA_CASE(2)
-        RTS_WTA_CASE(4)
-        RTS_WTA_CASE(8)
-        RTS_WTA_CASE(16)
-        default:
-            if (ppt == 3) {
-                k_wta<4, MAXIMIZE><<<H, WTA_THREADS, shm, s>>>(vol, dl, dr, mask, beat, W, H, D, tol);
-            } else if (ppt <= 8) {
-                k_wta<8, MAXIMIZE><<<H, WTA_THREADS, shm, s>>>(vol, dl, dr, mask, beat, W, H, D, tol);
-            } else {
-                k_wta<16, MAXIMIZE><<<H, WTA_THREADS, shm, s>>>(vol, dl, dr, mask, beat, W, H, D, tol);
-            }
-    }
-#undef RTS_WTA_CASE
+static int launch_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *beat, int W, int H, int D,
+                      hipStream_t s) {
+    RTS_CHECK(H <= 65535, "image height %d too large", H);
+    if (dr)
+        launch_wta_r<MAXIMIZE, true>(vol, dl, dr, beat, W, H, D, s);
+    else
+        launch_wta_r<MAXIMIZE, false>(vol, dl, dr, beat, W, H, D, s);
     RTS_LAUNCHED();
     return RTS_OK;
 }
@@ -256,7 +264,6 @@ rts_status rts_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, in
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(dl != nullptr, "rts_wta needs dl");
     RTS_CHECK(mask == nullptr || dr != nullptr, "the left-right mask needs dr");
-    RTS_CHECK(W <= 16 * WTA_THREADS, "row width %d exceeds %d", W, 16 * WTA_THREADS);
     const bool maximize = flags & RTS_WTA_MAX;
     const bool literal = flags & RTS_WTA_LITERAL;
     hipStream_t s = S(stream);
@@ -267,20 +274,19 @@ rts_status rts_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, in
         RTS_HIP(hipMallocAsync((void **)&beat, (size_t)W * H, s));
         RTS_HIP(hipMallocAsync((void **)&last, sizeof(int) * (size_t)H, s));
     }
-    int st = maximize ? launch_wta<true>(vol, dl, dr, mask, beat, W, H, D, tol, s)
-                      : launch_wta<false>(vol, dl, dr, mask, beat, W, H, D, tol, s);
+    int st = maximize ? launch_wta<true>(vol, dl, dr, beat, W, H, D, s) : launch_wta<false>(vol, dl, dr, beat, W, H, D, s);
     if (st == RTS_OK && literal) {
         k_carry_rowlast<<<div_up(H, 64), 64, 0, s>>>(dl, beat, last, W, H);
         k_carry_scan<<<1, 1, 0, s>>>(last, H);
         k_carry_fill<<<div_up(H, 64), 64, 0, s>>>(dl, beat, last, W, H);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) st = fail(RTS_ERR_HIP, "carry launch: %s", hipGetErrorString(e));
-        // The literal carry changes D_L after the fused check ran: redo it.
-        if (st == RTS_OK && mask) {
-            k_cross_check<<<dim3(div_up(W, 256), H), 256, 0, s>>>(dl, dr, mask, W, H, tol);
-            e = hipGetLastError();
-            if (e != hipSuccess) st = fail(RTS_ERR_HIP, "cross-check launch: %s", hipGetErrorString(e));
-        }
+    }
+    // the left-right check runs on the final D_L (after the literal carry)
+    if (st == RTS_OK && mask) {
+        k_cross_check<<<dim3(div_up(W, 256), H), 256, 0, s>>>(dl, dr, mask, W, H, tol);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = fail(RTS_ERR_HIP, "cross-check launch: %s", hipGetErrorString(e));
     }
     if (beat) (void)hipFreeAsync(beat, s);
     if (last) (void)hipFreeAsync(last, s);
