@@ -24,104 +24,159 @@ namespace rts {
 // values and a zero table (adding +0.0f is exact).
 //
 // One block per image row, one lane per disparity: lane d walks x = 0..W-1
-// serially, so its additions happen in the reference's order.  The right
-// view's census and colour rows sit in LDS (lanes read consecutive x-d);
-// so do the left view's, whose values at x are the same for every lane
-// (broadcast reads).  Each lane
-// stages FH_CH prefix values in LDS, then the wave stores its 64 rows of
-// FH_CH floats contiguously.
-constexpr int FH_CH = 32;
-constexpr int FH_MAXW = 4096;
+// serially, so its additions happen in the reference's order.  The left
+// view's census and packed colour at x are the same for every lane; they
+// arrive as vector loads of a lane-invariant address one 8-column chunk
+// ahead.  The right view's row and the two tables sit in LDS (about 12*W
+// bytes).  Columns are taken eight at a time: their costs first
+// (independent reads), then the serial adds; past the first D columns x - d
+// never wraps and the right view's eight reads are immediate offsets from
+// one address.  Each wave stages FC columns of its 64 lanes' prefix values in
+// its own LDS area and stores them as whole row segments (FC/4 lanes of
+// 16 bytes per row), synchronising within the wave only.  Rows whose width is
+// not a multiple of 4 (not 16-byte aligned) and the last W % FC columns run
+// column by column.
+constexpr int FH_MAXW = 4096;   // widest row a pipeline accepts
 
-template <bool CENSUS, bool AD>
+template <typename T>
+__device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
+    return *(const T *)((const char *)base + off);
+}
+
+// A value the compiler cannot see through (keeps a uniform offset in a VGPR).
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+template <bool CENSUS, bool AD, int FC>
 __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict__ cL, const uint64_t *__restrict__ cR,
-                                                     const uint8_t *__restrict__ rgbL, const uint8_t *__restrict__ rgbR,
+                                                     const uint32_t *__restrict__ aL, const uint32_t *__restrict__ aR,
                                                      const float *__restrict__ tabC, const float *__restrict__ tabA,
                                                      int nA, float *__restrict__ P, int W, int H, int D, int vec) {
     extern __shared__ uint8_t smem[];
-    uint64_t *rowC = reinterpret_cast<uint64_t *>(smem);                        // W (right census)
-    uint64_t *rowCL = rowC + (CENSUS ? W : 0);                                 // W (left census)
-    uint32_t *rowA = reinterpret_cast<uint32_t *>(rowCL + (CENSUS ? W : 0));  // W (right packed colour)
-    uint32_t *rowAL = rowA + (AD ? W : 0);                                     // W (left packed colour)
-    float *tC = reinterpret_cast<float *>(rowAL + (AD ? W : 0));              // 65
-    float *tA = tC + 65;                                                       // nA
-    float *stage = tA + nA;                                                    // waves * 64 * (FH_CH+1)
+    uint64_t *rowC = reinterpret_cast<uint64_t *>(smem);                      // W (right census)
+    uint32_t *rowA = reinterpret_cast<uint32_t *>(rowC + (CENSUS ? W : 0));  // W (right packed colour)
+    float *tC = reinterpret_cast<float *>(rowA + (AD ? W : 0));              // 65
+    float *tA = tC + 65;                                                      // nA
+    float *stage = tA + nA;                                                   // waves * 64 * (FC+1)
     const int y = blockIdx.x;
     const size_t roff = (size_t)y * W;
     for (int x = threadIdx.x; x < W; x += blockDim.x) {
-        if (CENSUS) {
-            rowC[x] = cR[roff + x];
-            rowCL[x] = cL[roff + x];
-        }
-        if (AD) {
-            rowA[x] = pack_rgb(rgbR + 3 * (roff + x));
-            rowAL[x] = pack_rgb(rgbL + 3 * (roff + x));
-        }
+        if (CENSUS) rowC[x] = cR[roff + x];
+        if (AD) rowA[x] = aR[roff + x];
     }
     for (int i = threadIdx.x; i < 65; i += blockDim.x) tC[i] = CENSUS ? tabC[i] : 0.0f;
     for (int i = threadIdx.x; i < nA; i += blockDim.x) tA[i] = tabA[i];
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int d = threadIdx.x;
-    float *st = stage + wave * 64 * (FH_CH + 1);
-    const size_t pl = plane(W, H);
-    float acc = 0.0f;
-    int xt = d == 0 ? 0 : W - d;   // (x - d) mod W at x = 0
-    for (int x0 = 0; x0 < W; x0 += FH_CH) {
-        const int n = min(FH_CH, W - x0);
-        if (d < D) {
-            // Costs of 8 columns first (independent loads), then the serial adds.
-            for (int j0 = 0; j0 < n; j0 += 8) {
-                float c[8];
+    // Lanes past D compute lane D-1's column (their values are never stored)
+    // so that every lane of a wave takes part in the wave's stores.
+    const int d = min((int)threadIdx.x, D - 1);
+    const uint64_t *cl = cL + roff;
+    const uint32_t *al = aL + roff;
+    float *Pd = P + (size_t)d * plane(W, H) + roff;
+    auto cost_lr = [&](uint64_t lc, uint32_t la, int xr) -> float {
+        if (CENSUS && AD) return tC[__popcll(lc ^ rowC[xr])] + tA[sad3(la, rowA[xr])];
+        if (CENSUS) return tC[__popcll(lc ^ rowC[xr])];
+        return tA[sad3(la, rowA[xr])];
+    };
+    // The left view's eight values of a chunk are loaded one chunk ahead as
+    // vector loads of a lane-invariant address (scalar loads would share
+    // their wait counter with the LDS reads).
+    uint64_t lcA[8], lcB[8];
+    uint32_t laA[8], laB[8];
+    auto load_left = [&](uint64_t(&lc)[8], uint32_t(&la)[8], int x0) {
+        const uint32_t oc = opaque_u32((uint32_t)x0 * 8u), oa = opaque_u32((uint32_t)x0 * 4u);
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int j = j0 + k;
-                    int xk = xt + k;
-                    xk = xk >= W ? xk - W : xk;
-                    const int x = min(x0 + j, W - 1);
-                    if (CENSUS && AD)
-                        c[k] = tC[__popcll(rowCL[x] ^ rowC[xk])] + tA[sad3(rowAL[x], rowA[xk])];
-                    else if (CENSUS)
-                        c[k] = tC[__popcll(rowCL[x] ^ rowC[xk])];
-                    else
-                        c[k] = tA[sad3(rowAL[x], rowA[xk])];
-                }
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int j = j0 + k;
-                    if (j < n) {
-                        acc = (x0 + j == 0) ? c[k] : acc + c[k];
-                        st[lane * (FH_CH + 1) + j] = acc;
-                    }
-                }
-                xt += 8;
-                xt = xt >= W ? xt - W : xt;
-            }
+        for (int q = 0; q < 4; q++) {
+            const uint4 v = CENSUS ? at_byte(reinterpret_cast<const uint4 *>(cl), oc + 16u * q) : make_uint4(0, 0, 0, 0);
+            lc[2 * q] = v.x | (uint64_t)v.y << 32;
+            lc[2 * q + 1] = v.z | (uint64_t)v.w << 32;
         }
-        __syncthreads();
-        // Store the wave's 64 rows (its lanes' disparities) of n floats.
-        const int dbase = wave * 64;
-        if (n == FH_CH && vec) {
-            // 8 lanes x float4 per row, 8 rows per instruction.
 #pragma unroll
-            for (int it = 0; it < 8; it++) {
-                const int rr = it * 8 + (lane >> 3), j = (lane & 7) * 4;
-                const int dd = dbase + rr;
-                if (dd < D) {
-                    const float *s4 = st + rr * (FH_CH + 1) + j;
-                    *reinterpret_cast<float4 *>(P + (size_t)dd * pl + roff + x0 + j) =
-                        make_float4(s4[0], s4[1], s4[2], s4[3]);
-                }
+        for (int q = 0; q < 2; q++) {
+            const uint4 v = AD ? at_byte(reinterpret_cast<const uint4 *>(al), oa + 16u * q) : make_uint4(0, 0, 0, 0);
+            la[4 * q] = v.x;
+            la[4 * q + 1] = v.y;
+            la[4 * q + 2] = v.z;
+            la[4 * q + 3] = v.w;
+        }
+    };
+    float acc = 0.0f;
+    const int lane = threadIdx.x & 63;
+    float *st = stage + (threadIdx.x >> 6) * 64 * (FC + 1);
+    // d plane of the wave's first lane: the wave stores rows dw .. dw+63
+    const int dw = threadIdx.x & ~63;
+    auto chunk = [&](const uint64_t(&lc)[8], const uint32_t(&la)[8], int x0) {
+        float c[8];
+        if (x0 < D) {
+            // x - d < 0 for lanes d > x: the reference wraps to x - d + W
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                int xr = x0 + k - d;
+                xr += xr < 0 ? W : 0;
+                c[k] = cost_lr(lc[k], la[k], xr);
             }
         } else {
-            for (int i = lane; i < 64 * n; i += 64) {
-                const int rr = i / n, j = i - rr * n;
-                const int dd = dbase + rr;
-                if (dd < D) P[(size_t)dd * pl + roff + x0 + j] = st[rr * (FH_CH + 1) + j];
-            }
+            const int xb = x0 - d;
+#pragma unroll
+            for (int k = 0; k < 8; k++) c[k] = cost_lr(lc[k], la[k], xb + k);
         }
-        __syncthreads();
+        float *sl = st + lane * (FC + 1) + (x0 & (FC - 1));
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            acc = (x0 + k == 0) ? c[k] : acc + c[k];
+            sl[k] = acc;
+        }
+        if ((x0 & (FC - 1)) == FC - 8) {
+            // the wave's 64 rows of FC prefix values leave as 16-byte stores,
+            // FC/4 lanes per row
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int xs = x0 - (FC - 8);
+            constexpr int LPR = FC / 4, RPI = 64 / LPR;   // lanes per row, rows per instruction
+#pragma unroll
+            for (int it = 0; it < LPR; it++) {
+                const int rr = it * RPI + lane / LPR, j = (lane % LPR) * 4;
+                const float *s4 = st + rr * (FC + 1) + j;
+                if (dw + rr < D)
+                    *reinterpret_cast<float4 *>(P + (size_t)(dw + rr) * plane(W, H) + roff + xs + j) =
+                        make_float4(s4[0], s4[1], s4[2], s4[3]);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    };
+    // The chunked path needs 16-byte aligned rows (W a multiple of 4) and
+    // covers whole FC-column strips; the rest runs column by column below.
+    const int Wc = vec ? W - W % FC : 0;
+    if (Wc > 0) load_left(lcA, laA, 0);
+    for (int x0 = 0; x0 < Wc; x0 += 16) {
+        if (x0 + 8 < Wc) load_left(lcB, laB, x0 + 8);
+        chunk(lcA, laA, x0);
+        if (x0 + 8 >= Wc) break;
+        if (x0 + 16 < Wc) load_left(lcA, laA, x0 + 16);
+        chunk(lcB, laB, x0 + 8);
     }
+    auto cost = [&](int x, int xr) -> float {
+        return cost_lr(CENSUS ? cl[x] : 0, AD ? al[x] : 0u, xr);
+    };
+    for (int x = Wc; x < W; x++) {
+        int xr = x - d;
+        xr += xr < 0 ? W : 0;
+        const float c = cost(x, xr);
+        acc = x == 0 ? c : acc + c;
+        if ((int)threadIdx.x < D) Pd[x] = acc;
+    }
+}
+
+// Packed 0x00BBGGRR colour per pixel, the layout sad3 and the fused front
+// end read.
+__global__ void k_pack_rgb(const uint8_t *__restrict__ rgb, uint32_t *__restrict__ out, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pack_rgb(rgb + 3 * i);
 }
 
 // The AD value the colour AD kernel produces for integer channel sum s.
@@ -144,6 +199,7 @@ struct rts_pipeline {
     // per-pixel buffers
     uint8_t *gray[2] = {nullptr, nullptr};
     uint64_t *census[2] = {nullptr, nullptr};
+    uint32_t *pcol[2] = {nullptr, nullptr};   // packed colour for the fused front end
     uint32_t *arms[2] = {nullptr, nullptr};
     uint8_t *rank[2] = {nullptr, nullptr};
     float *logimg[2] = {nullptr, nullptr};
@@ -259,6 +315,7 @@ size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
     size_t b = 2 * vol + 2 * P;   // two volumes, gray
     if (uses(*d, RTS_COST_CENSUS)) b += 2 * P * 8;
     if (d->aggr == RTS_AGGR_CROSS) b += 2 * P * 4;
+    if (d->aggr == RTS_AGGR_CROSS && uses(*d, RTS_COST_AD)) b += 2 * P * 4;   // packed colour (fused front end)
     if (uses(*d, RTS_COST_RANK)) b += 2 * P;
     if (uses(*d, RTS_COST_LOG)) b += 2 * P * 4;
     if (d->aggr == RTS_AGGR_AW) b += rts_aw_workspace_bytes(d->width, d->height, d->aw_r, d->aw_precision) + 6 * P;
@@ -297,6 +354,8 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
                           ((d.cost1 == RTS_COST_CENSUS && (d.cost2 == 0 || d.cost2 == RTS_COST_AD)) ||
                            (d.cost1 == RTS_COST_AD && (d.cost2 == 0 || d.cost2 == RTS_COST_CENSUS)));
     p->fused_front = front_ok;
+    if (front_ok && uses(d, RTS_COST_AD))
+        for (int v = 0; v < 2; v++) A((void **)&p->pcol[v], P * 4);
     if (front_ok && st == RTS_OK) {
         float hC[65], hA[766];
         const bool comb = d.cost2 != 0;
@@ -329,6 +388,7 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
     for (int v = 0; v < 2; v++) {
         (void)hipFree(p->gray[v]);
         (void)hipFree(p->census[v]);
+        (void)hipFree(p->pcol[v]);
         (void)hipFree(p->arms[v]);
         (void)hipFree(p->rank[v]);
         (void)hipFree(p->logimg[v]);
@@ -378,6 +438,32 @@ int build_cost(rts_pipeline *p, int which, const uint8_t *const rgb[2], const ui
 
 }  // namespace
 
+template <bool C, bool A, int FC>
+static int launch_front_fc(rts_pipeline *p, float *P, int W, int H, int D, int vec, hipStream_t s) {
+    const int waves = div_up(D, 64);
+    const size_t shm = (size_t)W * ((C ? 8 : 0) + (A ? 4 : 0)) + sizeof(float) * (65 + p->nA) +
+                       sizeof(float) * waves * 64 * (FC + 1);
+    RTS_CHECK(shm <= 160 * 1024, "front-end LDS %zu too large", shm);
+    if (shm > 65536) {   // the kernel's dynamic LDS above the default 64 KiB
+        static bool raised = false;
+        if (!raised) {
+            RTS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_front_hscan<C, A, FC>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            raised = true;
+        }
+    }
+    k_front_hscan<C, A, FC><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], p->pcol[0], p->pcol[1], p->tabC,
+                                                       p->tabA, p->nA, P, W, H, D, vec);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+// 32-column strips: each stored row segment is one 128-byte line.
+template <bool C, bool A>
+static int launch_front(rts_pipeline *p, float *P, int W, int H, int D, int vec, hipStream_t s) {
+    return launch_front_fc<C, A, 32>(p, P, W, H, D, vec, s);
+}
+
 extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
                                        const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
                                        const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask,
@@ -417,32 +503,17 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
         if (census)
             for (int v = 0; v < 2; v++)
                 if (int e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, s)) return e;
-        RTS_MARK(p, "prep", s);
-        const int waves = div_up(D, 64);
-        const size_t shm = (size_t)W * ((census ? 16 : 0) + (ad ? 8 : 0)) + sizeof(float) * (65 + p->nA) +
-                           sizeof(float) * waves * 64 * (FH_CH + 1);
-        RTS_CHECK(shm <= 160 * 1024, "front-end LDS %zu too large", shm);
-        if (shm > 65536) {   // the kernel's dynamic LDS above the default 64 KiB
-            static bool raised[3] = {false, false, false};
-            const int which = census && ad ? 0 : census ? 1 : 2;
-            if (!raised[which]) {
-                const void *fn = census && ad ? reinterpret_cast<const void *>(&k_front_hscan<true, true>)
-                                 : census     ? reinterpret_cast<const void *>(&k_front_hscan<true, false>)
-                                              : reinterpret_cast<const void *>(&k_front_hscan<false, true>);
-                RTS_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-                raised[which] = true;
+        if (ad)
+            for (int v = 0; v < 2; v++) {
+                k_pack_rgb<<<(unsigned)div_up((long)W * H, 256), 256, 0, s>>>(rgb[v], p->pcol[v], (size_t)W * H);
+                RTS_LAUNCHED();
             }
-        }
+        RTS_MARK(p, "prep", s);
         const int vec = (W % 4) == 0 && ((uintptr_t)cur % 16) == 0;
-        if (census && ad)
-            k_front_hscan<true, true><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
-                                                                 p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
-        else if (census)
-            k_front_hscan<true, false><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
-                                                                  p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
-        else
-            k_front_hscan<false, true><<<H, waves * 64, shm, s>>>(p->census[0], p->census[1], rgb[0], rgb[1],
-                                                                  p->tabC, p->tabA, p->nA, cur, W, H, D, vec);
+        int e = census && ad ? launch_front<true, true>(p, cur, W, H, D, vec, s)
+                : census     ? launch_front<true, false>(p, cur, W, H, D, vec, s)
+                             : launch_front<false, true>(p, cur, W, H, D, vec, s);
+        if (e) return e;
         RTS_LAUNCHED();
         prefixed = true;
     } else if (d.cost3) {
