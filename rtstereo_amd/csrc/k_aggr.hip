@@ -5,6 +5,8 @@
 
 #include <stdlib.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace rts {
@@ -195,6 +197,55 @@ __global__ void __launch_bounds__(256) k_find_cross_tile(const uint8_t *__restri
     };
     const uint32_t l = len(0, -1), r = len(0, 1), t = len(-1, 0), b = len(1, 0);
     arms[(size_t)y * W + x] = l | (r << 8) | (t << 16) | (b << 24);
+}
+
+// The same with 16 output rows per block (4 per thread), so that the
+// 34-row halo is staged once per 16 rows; PACKED reads the packed colour
+// image (one load per staged pixel), otherwise the interleaved bytes.
+constexpr int FC_TY = 16;
+template <bool PACKED>
+__global__ void __launch_bounds__(256) k_find_cross_t(const void *__restrict__ src, uint32_t *__restrict__ arms,
+                                                      int W, int H, int L, int tao) {
+    constexpr int TH = FC_TY + 2 * FC_L, TW = 64 + 2 * FC_L;
+    __shared__ uint32_t tile[TH][TW + 1];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * FC_TY;
+    for (int i = threadIdx.y * 64 + threadIdx.x; i < TH * TW; i += 256) {
+        const int r = i / TW, c = i - r * TW;
+        const int yy = y0 - FC_L + r, xx = x0 - FC_L + c;
+        uint32_t v = 0;
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+            const size_t q = (size_t)yy * W + xx;
+            v = PACKED ? static_cast<const uint32_t *>(src)[q] : pack_rgb(static_cast<const uint8_t *>(src) + 3 * q);
+        }
+        tile[r][c] = v;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
+    if (x >= W) return;
+    const int tx = threadIdx.x + FC_L;
+    for (int ly = threadIdx.y; ly < FC_TY; ly += 4) {
+        const int y = y0 + ly;
+        if (y >= H) break;
+        const int ty = ly + FC_L;
+        const uint32_t c = tile[ty][tx];
+        // arm length: steps while the previous step's delta is below tao and
+        // the arm is shorter than L; the step reaching tao is included; the
+        // walk stops at the image edge, dist pixels away; a zero arm with an
+        // in-image first neighbour becomes 1
+        auto len = [&](int sy, int sx, int dist) {
+            const int lim = min(L, dist);
+            int n = 0, delta = 0;
+            while (delta < tao && n < lim) {
+                n++;
+                delta = max_chan_delta(c, tile[ty + sy * n][tx + sx * n]);
+            }
+            if (n == 0 && dist > 0) n = 1;
+            return (uint32_t)n;
+        };
+        const uint32_t l = len(0, -1, x), r = len(0, 1, W - 1 - x);
+        const uint32_t t = len(-1, 0, y), b = len(1, 0, H - 1 - y);
+        arms[(size_t)y * W + x] = l | (r << 8) | (t << 16) | (b << 24);
+    }
 }
 
 __global__ void k_arms_unpack(const uint32_t *__restrict__ arms, int *__restrict__ horz, int *__restrict__ vert, int P) {
@@ -664,14 +715,23 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
     const int xc = min(x, W - 1);
     const int tx = wrap1(xc - d, W);
     const int cs = x0 - CV_LMAX - 1;
-    const int c0 = cs + lane, c1 = cs + 64 + lane;
-    const bool ok0 = c0 >= 0 && c0 < W, ok1 = lane < CV_COLS - 64 && c1 >= 0 && c1 < W;
+    // The 99 prefix values a row's segments can touch (columns cs .. cs+98)
+    // are held as two registers per lane: lo = column cs+j on lane j, and
+    // hi = column cs+64+j on lanes j < 47 (cs+64 .. cs+98 on j <= 34), column
+    // cs+j on lanes j >= 47.  A left end cs+jl with jl in [lane, lane+17]
+    // then lies in lo for lanes below 47 and in hi (source lane jl mod 64)
+    // for the others; a right end jr in [lane+18, lane+35] lies in lo for
+    // lanes below 29 and in hi for the others, so the register is a fixed
+    // choice per lane.
+    const int c0 = cs + lane, c1 = cs + lane + (lane < 47 ? 64 : 0);
+    const bool ok0 = c0 >= 0 && c0 < W, ok1 = (lane < CV_COLS - 64 || lane >= 47) && c1 >= 0 && c1 < W;
+    // blocks whose 99 columns all lie inside the image need no clamping
+    const bool interior = cs >= 0 && cs + CV_COLS <= W;
     const float *Pb = P + base;
     // uniform row bases with 32-bit per-lane byte offsets (scalar-base addressing)
     const uint32_t oxc = (uint32_t)xc * 4u, otx = (uint32_t)tx * 4u;
     const uint32_t oc0 = (uint32_t)c0 * 4u, oc1 = (uint32_t)c1 * 4u;
-    // Per row and lane: the 99 prefix values of the row's segments held as
-    // two registers (columns cs+lane and cs+64+lane), and the two arm words.
+    const bool hi_l = lane >= 47, hi_r = lane >= 29;
     // A step's loads are issued one step before its segment sums are formed.
     float p0[KR], p1[KR];
     uint32_t ar[KR], at[KR];
@@ -688,20 +748,28 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
     int out_next = s0;
     int pr = 0;   // ring position of row r: (r - rlo) mod R
     load_rows(rlo);
-    for (int r = rlo; r < rhi; r += CH) {
+    // segment sums of one step's rows into the ring; INTERIOR: every column
+    // of the block's window lies in the image (no clamps, fixed registers)
+    auto produce = [&](auto interior_tag, int r) {
+        constexpr bool INTERIOR = decltype(interior_tag)::value;
 #pragma unroll
         for (int k = 0; k < KR; k++) {
             const uint32_t a = ar[k], c = at[k];
             const uint32_t l = min(a & 0xFFu, c & 0xFFu), rr = min((a >> 8) & 0xFFu, (c >> 8) & 0xFFu);
             const uint32_t t = min((a >> 16) & 0xFFu, (c >> 16) & 0xFFu), b = min(a >> 24, c >> 24);
-            // columns of the segment's ends relative to cs (ds_bpermute takes
-            // the source lane from address bits 7:2, so j*4 selects lane j % 64)
-            const int jl = max(xc - (int)l - 1, 0) - cs, jr = min(xc + (int)rr, W - 1) - cs;
-            const int bl = __builtin_amdgcn_ds_bpermute(jl * 4, __float_as_int(p0[k]));
-            const int bh = __builtin_amdgcn_ds_bpermute(jl * 4, __float_as_int(p1[k]));
-            const int el = __builtin_amdgcn_ds_bpermute(jr * 4, __float_as_int(p0[k]));
-            const int eh = __builtin_amdgcn_ds_bpermute(jr * 4, __float_as_int(p1[k]));
-            const float sv = __int_as_float(jr < 64 ? el : eh) - __int_as_float(jl < 64 ? bl : bh);
+            // byte addresses of the segment's ends relative to cs (ds_bpermute
+            // takes the source lane from address bits 7:2, so 4j selects lane
+            // j % 64)
+            const int jl4 = INTERIOR ? 4 * (lane + CV_LMAX) - 4 * (int)l : 4 * (max(xc - (int)l - 1, 0) - cs);
+            const int jr4 = INTERIOR ? 4 * (int)rr + 4 * (lane + CV_LMAX + 1) : 4 * (min(xc + (int)rr, W - 1) - cs);
+            const int bl = __builtin_amdgcn_ds_bpermute(jl4, __float_as_int(p0[k]));
+            const int bh = __builtin_amdgcn_ds_bpermute(jl4, __float_as_int(p1[k]));
+            const int el = __builtin_amdgcn_ds_bpermute(jr4, __float_as_int(p0[k]));
+            const int eh = __builtin_amdgcn_ds_bpermute(jr4, __float_as_int(p1[k]));
+            // hi holds column cs+64+j for j < 47: a clamped end below 64
+            // reads lo, one at or above 64 reads hi
+            const bool uh_l = INTERIOR ? hi_l : jl4 >= 256, uh_r = INTERIOR ? hi_r : jr4 >= 256;
+            const float sv = __int_as_float(uh_r ? eh : el) - __int_as_float(uh_l ? bh : bl);
             if (r + w + NW * k < rhi) {
                 const float2 v = make_float2(sv, __uint_as_float((l + rr + 1) | t << 16 | b << 24));
                 int p = pr + w + NW * k;
@@ -710,6 +778,12 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
                 if (p < 2 * CV_LMAX) ring[p + R][lane] = v;
             }
         }
+    };
+    for (int r = rlo; r < rhi; r += CH) {
+        if (interior)
+            produce(std::true_type{}, r);
+        else
+            produce(std::false_type{}, r);
         __syncthreads();
         const int rnext = r + CH;
         if (rnext < rhi) load_rows(rnext);
@@ -717,32 +791,28 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
         const int out_end = avail >= rhi ? s1 : min(s1, avail - CV_LMAX);
         if (colok) {
             // ring position of row y-17 for the wave's first output row (rows
-            // below rlo never enter a support), then advanced with y
+            // below rlo never enter a support), then advanced with y; each
+            // output row's entry word is read one iteration ahead
             int bm = (out_next + w - CV_LMAX - rlo + R) % R;
+            uint32_t word = __float_as_uint(ring[bm + CV_LMAX][lane].y);
             for (int y = out_next + w; y < out_end; y += NW) {
                 const float2 *col = &ring[bm][lane];
                 bm += NW;
                 if (bm >= R) bm -= R;
-                const uint32_t word = __float_as_uint(col[CV_LMAX * CV_TX].y);
+                const uint32_t next = __float_as_uint(ring[bm + CV_LMAX][lane].y);
                 const int t = (word >> 16) & 0xFF, b = word >> 24;
                 const int n = t + b + 1, lead = n & 3, q = n >> 2;
                 const float2 *s = col + (CV_LMAX - t) * CV_TX;
                 float sum = 0.0f;
                 uint32_t cw = 0;
                 if (lead) {
-                    float2 e = s[0];
-                    sum += e.x;
-                    cw += __float_as_uint(e.y);
-                    if (lead > 1) {
-                        e = s[CV_TX];
-                        sum += e.x;
-                        cw += __float_as_uint(e.y);
-                        if (lead > 2) {
-                            e = s[2 * CV_TX];
-                            sum += e.x;
-                            cw += __float_as_uint(e.y);
-                        }
-                    }
+                    // the lead rows in one round trip; a row past the lead adds
+                    // +0.0, which leaves the (never -0.0) sum unchanged
+                    const float2 a0 = s[0], a1 = s[CV_TX], a2 = s[2 * CV_TX];
+                    const bool u1 = lead > 1, u2 = lead > 2;
+                    sum = ((sum + a0.x) + (u1 ? a1.x : 0.0f)) + (u2 ? a2.x : 0.0f);
+                    cw = __float_as_uint(a0.y) + (u1 ? __float_as_uint(a1.y) : 0u) +
+                         (u2 ? __float_as_uint(a2.y) : 0u);
                 }
                 s += lead * CV_TX;
 #pragma unroll
@@ -750,13 +820,11 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
                     if (gi >= q) break;
                     const float2 e0 = s[(4 * gi) * CV_TX], e1 = s[(4 * gi + 1) * CV_TX];
                     const float2 e2 = s[(4 * gi + 2) * CV_TX], e3 = s[(4 * gi + 3) * CV_TX];
-                    sum += e0.x;
-                    sum += e1.x;
-                    sum += e2.x;
-                    sum += e3.x;
+                    sum = (((sum + e0.x) + e1.x) + e2.x) + e3.x;
                     cw += __float_as_uint(e0.y) + __float_as_uint(e1.y) + __float_as_uint(e2.y) +
                           __float_as_uint(e3.y);
                 }
+                word = next;
                 float *orow = out + base + (size_t)y * W;
                 *(float *)((char *)orow + (uint32_t)x * 4u) = div_count(sum, cw & 0xFFFFu);
             }
@@ -782,6 +850,16 @@ static int xs_segments(int W, int H, int D) {
     const long want = (4096L + strips - 1) / strips;
     return (int)(want < most ? want : most);
 }
+
+namespace rts {
+int find_cross_packed(const uint32_t *pcol, uint32_t *arms, int W, int H, int L, int tao, hipStream_t s) {
+    RTS_CHECK(W >= 1 && H >= 1, "bad image size %dx%d", W, H);
+    RTS_CHECK(L >= 0 && L <= FC_L, "arm limit %d must be in [0, %d]", L, FC_L);
+    k_find_cross_t<true><<<dim3(div_up(W, 64), div_up(H, FC_TY)), dim3(64, 4), 0, s>>>(pcol, arms, W, H, L, tao);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+}  // namespace rts
 
 template <bool DIRECT>
 static int launch_cross_tile(const float *in, float *out, const uint32_t *ar, const uint32_t *at, int W, int H, int D,
@@ -853,7 +931,7 @@ rts_status rts_find_cross(const uint8_t *rgb, uint32_t *arms, int W, int H, int 
     RTS_CHECK(L >= 0 && L <= 255, "arm limit %d must be in [0, 255]", L);
     dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
     if (L <= FC_L)
-        k_find_cross_tile<<<grid, block, 0, S(stream)>>>(rgb, arms, W, H, L, tao);
+        k_find_cross_t<false><<<dim3(div_up(W, 64), div_up(H, FC_TY)), block, 0, S(stream)>>>(rgb, arms, W, H, L, tao);
     else
         k_find_cross<<<grid, block, 0, S(stream)>>>(rgb, arms, W, H, L, tao);
     RTS_LAUNCHED();

@@ -117,6 +117,76 @@ __global__ void __launch_bounds__(256) k_census_tile(const uint8_t *__restrict__
     if (k) words[w * pl + (size_t)y * W + x] = acc;
 }
 
+// Census windows with wr <= 4 (the reference's 7x9 among them), four
+// pixels per thread: the pixels x = 4i..4i+3 of a row see columns
+// 4i-wr .. 4i+3+wr, all inside the three aligned dwords 4i-4 .. 4i+7 of the
+// staged row, so every byte a comparison reads is a fixed byte of a register
+// (byte-select compares).  Each comparison shifts its bit into the word with
+// an add-with-carry, taken from the last window entry down so that entry k
+// lands on bit k.  A block covers 256 columns x CW_TY rows; the staged rows
+// wrap around the image edges like the loop form.
+constexpr int CW_TY = 16;
+
+template <int HR, int WR>
+__global__ void __launch_bounds__(256) k_census_w4(const uint8_t *__restrict__ img, uint64_t *__restrict__ words,
+                                                   int W, int H) {
+    static_assert(WR <= 4 && (2 * HR + 1) * (2 * WR + 1) <= 64, "window");
+    constexpr int TR = CW_TY + 2 * HR, TWD = 66;   // staged rows, dwords per row (columns x0-4 .. x0+259)
+    __shared__ uint32_t tile[TR][TWD];
+    const int x0 = blockIdx.x * 256, y0 = blockIdx.y * CW_TY;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const bool inner = (W % 4) == 0 && x0 >= 4 && x0 + 260 <= W;
+    for (int i = tid; i < TR * TWD; i += 256) {
+        const int r = i / TWD, c = i - r * TWD;
+        const size_t ro = (size_t)wrapn(y0 - HR + r, H) * W;
+        uint32_t v;
+        if (inner) {
+            v = *reinterpret_cast<const uint32_t *>(img + ro + x0 - 4 + 4 * c);
+        } else {
+            v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) v |= (uint32_t)img[ro + wrapn(x0 - 4 + 4 * c + k, W)] << (8 * k);
+        }
+        tile[r][c] = v;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x, xb = x0 + 4 * tx;
+    if (xb >= W) return;
+    const size_t pl = plane(W, H);
+    for (int ty = threadIdx.y; ty < CW_TY; ty += 4) {
+        const int y = y0 + ty;
+        if (y >= H) break;
+        // dwords tx .. tx+2 of rows ty .. ty+2*HR: byte (4 + j + dx) of the
+        // three is the neighbour dx of pixel j
+        uint32_t rows[2 * HR + 1][3];
+#pragma unroll
+        for (int r = 0; r <= 2 * HR; r++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) rows[r][q] = tile[ty + r][tx + q];
+        uint64_t out[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t c = (rows[HR][(4 + j) >> 2] >> (8 * ((4 + j) & 3))) & 255u;
+            uint32_t lo = 0, hi = 0;
+#pragma unroll
+            for (int k = (2 * HR + 1) * (2 * WR + 1) - 1; k >= 0; k--) {
+                const int dy = k / (2 * WR + 1), dx = k % (2 * WR + 1) - WR;
+                const int bpos = 4 + j + dx;
+                const uint32_t nb = (rows[dy][bpos >> 2] >> (8 * (bpos & 3))) & 255u;
+                const uint32_t bit = c > nb;
+                if (k >= 32)
+                    hi = hi + hi + bit;
+                else
+                    lo = lo + lo + bit;
+            }
+            out[j] = (uint64_t)hi << 32 | lo;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (xb + j < W) words[(size_t)y * W + xb + j] = out[j];
+    }
+}
+
 __global__ void k_census_unpack(const uint64_t *__restrict__ words, uint8_t *__restrict__ bools, int P, int N) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
@@ -545,7 +615,9 @@ rts_status rts_census(const uint8_t *img, uint64_t *words, int W, int H, int hr,
     RTS_CHECK(W >= 1 && H >= 1, "bad image size %dx%d", W, H);
     RTS_CHECK(hr >= 0 && wr >= 0 && hr < H && wr < W, "census radii %d,%d do not fit the image", hr, wr);
     dim3 grid(div_up(W, 64), div_up(H, 4)), block(64, 4);
-    if (hr <= CT_R && wr <= CT_R)
+    if (hr == 4 && wr == 3)
+        k_census_w4<4, 3><<<dim3(div_up(W, 256), div_up(H, CW_TY)), block, 0, S(stream)>>>(img, words, W, H);
+    else if (hr <= CT_R && wr <= CT_R)
         k_census_tile<<<grid, block, 0, S(stream)>>>(img, words, W, H, hr, wr);
     else
         k_census<<<grid, block, 0, S(stream)>>>(img, words, W, H, hr, wr);

@@ -199,7 +199,7 @@ struct rts_pipeline {
     // per-pixel buffers
     uint8_t *gray[2] = {nullptr, nullptr};
     uint64_t *census[2] = {nullptr, nullptr};
-    uint32_t *pcol[2] = {nullptr, nullptr};   // packed colour for the fused front end
+    uint32_t *pcol[2] = {nullptr, nullptr};   // packed colour (cross arms, fused front end)
     uint32_t *arms[2] = {nullptr, nullptr};
     uint8_t *rank[2] = {nullptr, nullptr};
     float *logimg[2] = {nullptr, nullptr};
@@ -315,7 +315,7 @@ size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
     size_t b = 2 * vol + 2 * P;   // two volumes, gray
     if (uses(*d, RTS_COST_CENSUS)) b += 2 * P * 8;
     if (d->aggr == RTS_AGGR_CROSS) b += 2 * P * 4;
-    if (d->aggr == RTS_AGGR_CROSS && uses(*d, RTS_COST_AD)) b += 2 * P * 4;   // packed colour (fused front end)
+    if (d->aggr == RTS_AGGR_CROSS) b += 2 * P * 4;   // packed colour
     if (uses(*d, RTS_COST_RANK)) b += 2 * P;
     if (uses(*d, RTS_COST_LOG)) b += 2 * P * 4;
     if (d->aggr == RTS_AGGR_AW) b += rts_aw_workspace_bytes(d->width, d->height, d->aw_r, d->aw_precision) + 6 * P;
@@ -354,7 +354,7 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
                           ((d.cost1 == RTS_COST_CENSUS && (d.cost2 == 0 || d.cost2 == RTS_COST_AD)) ||
                            (d.cost1 == RTS_COST_AD && (d.cost2 == 0 || d.cost2 == RTS_COST_CENSUS)));
     p->fused_front = front_ok;
-    if (front_ok && uses(d, RTS_COST_AD))
+    if (d.aggr == RTS_AGGR_CROSS)
         for (int v = 0; v < 2; v++) A((void **)&p->pcol[v], P * 4);
     if (front_ok && st == RTS_OK) {
         float hC[65], hA[766];
@@ -495,19 +495,22 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
     }
     float *cur = p->vol[0], *other = p->vol[1];
     if (d.aggr == RTS_AGGR_CROSS)
-        for (int v = 0; v < 2; v++)
-            if (int e = rts_find_cross(rgb[v], p->arms[v], W, H, d.cross_L, d.cross_tao, s)) return e;
+        for (int v = 0; v < 2; v++) {
+            k_pack_rgb<<<(unsigned)div_up((long)W * H, 256), 256, 0, s>>>(rgb[v], p->pcol[v], (size_t)W * H);
+            RTS_LAUNCHED();
+            if (d.cross_L <= 17) {
+                if (int e = find_cross_packed(p->pcol[v], p->arms[v], W, H, d.cross_L, d.cross_tao, s)) return e;
+            } else if (int e = rts_find_cross(rgb[v], p->arms[v], W, H, d.cross_L, d.cross_tao, s)) {
+                return e;
+            }
+        }
     bool prefixed = false;
     if (p->fused_front) {
         const bool census = uses(d, RTS_COST_CENSUS), ad = uses(d, RTS_COST_AD);
         if (census)
             for (int v = 0; v < 2; v++)
                 if (int e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, s)) return e;
-        if (ad)
-            for (int v = 0; v < 2; v++) {
-                k_pack_rgb<<<(unsigned)div_up((long)W * H, 256), 256, 0, s>>>(rgb[v], p->pcol[v], (size_t)W * H);
-                RTS_LAUNCHED();
-            }
+        // the packed colour images were made with the cross arms above
         RTS_MARK(p, "prep", s);
         const int vec = (W % 4) == 0 && ((uintptr_t)cur % 16) == 0;
         int e = census && ad ? launch_front<true, true>(p, cur, W, H, D, vec, s)

@@ -71,6 +71,9 @@ __host__ __device__ __forceinline__ size_t plane(int W, int H) { return (size_t)
 // Validates W, H, D against the limits every entry point shares.
 int check_dims(int W, int H, int D);
 
+// Cross arms (arm limit at most 17) from a packed 0x00BBGGRR colour image.
+int find_cross_packed(const uint32_t *pcol, uint32_t *arms, int W, int H, int L, int tao, hipStream_t s);
+
 // Colour AD helper: |a-b| summed over the three bytes of packed 0x00BBGGRR
 // pixels (v_sad_u8 does four byte |differences| plus an accumulator).
 __device__ __forceinline__ uint32_t sad3(uint32_t a, uint32_t b) { return __builtin_amdgcn_sad_u8(a, b, 0u); }

@@ -54,6 +54,16 @@ def test_census_hamming(W, H, D, seed, hr, wr):
     assert np.array_equal(pm(S.hamming(wl, wr_, D)), oracle.hamming(bl, br, hr, wr, D))
 
 
+@pytest.mark.parametrize("W,H", [(1024, 40), (1030, 37), (520, 17)])
+def test_census_wide(W, H):
+    """Rows wide enough for the census kernel's aligned interior blocks (and
+    widths that are not a multiple of 4, which stage every row byte-wise),
+    with the pipeline's 7x9 window."""
+    p = _pair(W, H, 8, 41)
+    gl, _ = p.gray()
+    assert np.array_equal(S.census(gl, 4, 3), S.census_pack(oracle.census(gl, 4, 3)))
+
+
 @pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
 def test_rank_log(W, H, D, seed):
     p = _pair(W, H, D, seed)
