@@ -695,15 +695,22 @@ template <int CH, int NW>
 __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict__ P, float *__restrict__ out,
                                                          const uint32_t *__restrict__ aref,
                                                          const uint32_t *__restrict__ atar, int W, int H, int nx,
-                                                         int nseg, int seg, int D) {
+                                                         int nseg, int seg, int D, int xg) {
     constexpr int R = CH + 2 * CV_LMAX;        // rows live at once
     constexpr int RR = R + 2 * CV_LMAX;        // stored rows: the first 34 repeated past the end
     constexpr int KR = CH / NW;                // rows per wave per step
     static_assert(KR * NW == CH, "step rows must split evenly over the waves");
     __shared__ float2 ring[RR][CV_TX];
+    // Work order: row segment, then groups of xg column tiles, then d, then
+    // the tile within its group; cut into one contiguous range per XCD, so an
+    // XCD's blocks share the arm maps of a few tiles across many d (and the
+    // prefix columns of neighbouring tiles of one d) in its L2.
     const int wk = xcd_order(blockIdx.x, nx * nseg * D);
-    const int xt = wk % nx, rest = wk / nx;
-    const int d = rest % D, sg = rest / D;
+    const int per_seg = nx * D;
+    const int sg = wk / per_seg, wi = wk - sg * per_seg;
+    const int g = wi / (xg * D), wj = wi - g * (xg * D);
+    const int gsz = min(xg, nx - g * xg);
+    const int d = wj / gsz, xt = g * xg + (wj - d * gsz);
     const int x0 = xt * CV_TX, s0 = sg * seg;
     const int s1 = min(H, s0 + seg);
     const size_t base = (size_t)d * plane(W, H);
@@ -842,8 +849,11 @@ using namespace rts;
 
 // Shape of the short-arm sweep: 32 rows per step over 8 waves; a block owns
 // a column strip of XS_SEG_MIN.. rows, cut so that the grid holds about 4096
-// blocks (whole columns for 1080p at D >= 72).
-constexpr int XS_CH = 32, XS_NW = 8, XS_SEG_MIN = 64;
+// blocks (whole columns for 1080p at D >= 72).  Work groups of XS_GROUP
+// column tiles: an XCD's blocks walk every d of about four tiles, which
+// halves the sweep's fetches from memory (the arm maps stay in that XCD's
+// L2) at the same speed.
+constexpr int XS_CH = 32, XS_NW = 8, XS_SEG_MIN = 64, XS_GROUP = 4;
 static int xs_segments(int W, int H, int D) {
     const long strips = (long)div_up(W, CV_TX) * D;
     const long most = H / XS_SEG_MIN > 1 ? H / XS_SEG_MIN : 1;
@@ -974,7 +984,7 @@ rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, con
         const int nsg = xs_segments(W, H, D), seg = div_up(H, nsg);
         RTS_CHECK((long)nx * nsg * D < (1L << 31), "volume too large");
         k_cross_sweep<XS_CH, XS_NW><<<nx * nsg * D, 64 * XS_NW, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx,
-                                                                               nsg, seg, D);
+                                                                               nsg, seg, D, XS_GROUP);
     } else {
         k_cross_strip<<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
     }
