@@ -36,8 +36,13 @@ template <int PPT, bool MAXIMIZE, bool WANT_R, int DU, int T>
 __global__ void __launch_bounds__(T) k_wta(const float *__restrict__ vol, uint8_t *__restrict__ dl,
                                                      uint8_t *__restrict__ dr, uint8_t *__restrict__ beat0, int W,
                                                      int H, int D) {
-    const int y = blockIdx.y;
-    const int xb = blockIdx.x * (T * PPT) + threadIdx.x;
+    // Chunks of a row are consecutive in the work order and each XCD takes
+    // one contiguous range of it, so the right view's reads past the chunk
+    // (columns up to x + D - 1) find the next chunk's lines in the same L2.
+    const int nch = gridDim.x;
+    const int wk = xcd_order(blockIdx.y * nch + blockIdx.x, nch * gridDim.y);
+    const int y = wk / nch;
+    const int xb = (wk - y * nch) * (T * PPT) + threadIdx.x;
     const size_t pl = plane(W, H), rowoff = (size_t)y * W;
     const float *row = vol + rowoff;
     // byte offsets of the thread's pixels, and of the same pixels minus a row
