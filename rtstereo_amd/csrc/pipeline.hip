@@ -32,8 +32,9 @@ namespace rts {
 // (independent reads), then the serial adds; past the first D columns x - d
 // never wraps and the right view's eight reads are immediate offsets from
 // one address.  Each wave stages FC columns of its 64 lanes' prefix values in
-// its own LDS area and stores them as whole row segments (FC/4 lanes of
-// 16 bytes per row), synchronising within the wave only.  Rows whose width is
+// its own LDS area (16-byte pieces, swizzled against bank conflicts) and
+// stores them as whole row segments (FC/4 lanes of 16 bytes per row, four
+// rows' reads ahead of their stores), synchronising within the wave only.  Rows whose width is
 // not a multiple of 4 (not 16-byte aligned) and the last W % FC columns run
 // column by column.
 constexpr int FH_MAXW = 4096;   // widest row a pipeline accepts
@@ -59,7 +60,10 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
     uint32_t *rowA = reinterpret_cast<uint32_t *>(rowC + (CENSUS ? W : 0));  // W (right packed colour)
     float *tC = reinterpret_cast<float *>(rowA + (AD ? W : 0));              // 65
     float *tA = tC + 65;                                                      // nA
-    float *stage = tA + nA;                                                   // waves * 64 * (FC+1)
+    // stage: per wave 64 rows of FC floats, 16-byte groups XOR-swizzled by
+    // row (see stage_at), 16-byte aligned
+    const int soff = (((int)(reinterpret_cast<uint8_t *>(tA + nA) - smem)) + 15) & ~15;
+    float *stage = reinterpret_cast<float *>(smem + soff);                    // waves * 64 * FC
     const int y = blockIdx.x;
     const size_t roff = (size_t)y * W;
     for (int x = threadIdx.x; x < W; x += blockDim.x) {
@@ -104,7 +108,11 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
     };
     float acc = 0.0f;
     const int lane = threadIdx.x & 63;
-    float *st = stage + (threadIdx.x >> 6) * 64 * (FC + 1);
+    float *st = stage + (threadIdx.x >> 6) * 64 * FC;
+    // 16-byte group g of stage row r sits at group g ^ ((r ^ r >> 3) & 7):
+    // sixteen rows writing one group, and two rows read as eight groups
+    // each, then fall on distinct banks
+    auto stage_at = [&](int r, int g) { return st + r * FC + 4 * (g ^ ((r ^ (r >> 3)) & 7)); };
     // d plane of the wave's first lane: the wave stores rows dw .. dw+63
     const int dw = threadIdx.x & ~63;
     auto chunk = [&](const uint64_t(&lc)[8], const uint32_t(&la)[8], int x0) {
@@ -122,12 +130,15 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
 #pragma unroll
             for (int k = 0; k < 8; k++) c[k] = cost_lr(lc[k], la[k], xb + k);
         }
-        float *sl = st + lane * (FC + 1) + (x0 & (FC - 1));
+        float o[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             acc = (x0 + k == 0) ? c[k] : acc + c[k];
-            sl[k] = acc;
+            o[k] = acc;
         }
+        const int g0 = (x0 & (FC - 1)) >> 2;
+        *reinterpret_cast<float4 *>(stage_at(lane, g0)) = make_float4(o[0], o[1], o[2], o[3]);
+        *reinterpret_cast<float4 *>(stage_at(lane, g0 + 1)) = make_float4(o[4], o[5], o[6], o[7]);
         if ((x0 & (FC - 1)) == FC - 8) {
             // the wave's 64 rows of FC prefix values leave as 16-byte stores,
             // FC/4 lanes per row
@@ -136,13 +147,32 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int xs = x0 - (FC - 8);
             constexpr int LPR = FC / 4, RPI = 64 / LPR;   // lanes per row, rows per instruction
+            static_assert(LPR % 4 == 0, "stores go in groups of four");
+            // stage row rr = it*RPI + lane/LPR, columns j .. j+3: four rows'
+            // reads are issued before the first of their stores waits
+            float *Pw = P + (size_t)(dw + lane / LPR) * plane(W, H) + roff + xs + (lane % LPR) * 4;
+            const size_t pstep = (size_t)RPI * plane(W, H);
+            auto rd = [&](int it) {
+                return *reinterpret_cast<const float4 *>(stage_at(it * RPI + lane / LPR, lane % LPR));
+            };
+            auto wr = [&](int it, const float4 &v, bool guard) {
+                if (!guard || dw + it * RPI + lane / LPR < D) *reinterpret_cast<float4 *>(Pw + it * pstep) = v;
+            };
+            const bool guard = dw + 64 > D;
 #pragma unroll
-            for (int it = 0; it < LPR; it++) {
-                const int rr = it * RPI + lane / LPR, j = (lane % LPR) * 4;
-                const float *s4 = st + rr * (FC + 1) + j;
-                if (dw + rr < D)
-                    *reinterpret_cast<float4 *>(P + (size_t)(dw + rr) * plane(W, H) + roff + xs + j) =
-                        make_float4(s4[0], s4[1], s4[2], s4[3]);
+            for (int h = 0; h < LPR; h += 4) {
+                const float4 v0 = rd(h), v1 = rd(h + 1), v2 = rd(h + 2), v3 = rd(h + 3);
+                if (!guard) {
+                    wr(h, v0, false);
+                    wr(h + 1, v1, false);
+                    wr(h + 2, v2, false);
+                    wr(h + 3, v3, false);
+                } else {
+                    wr(h, v0, true);
+                    wr(h + 1, v1, true);
+                    wr(h + 2, v2, true);
+                    wr(h + 3, v3, true);
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -441,8 +471,8 @@ int build_cost(rts_pipeline *p, int which, const uint8_t *const rgb[2], const ui
 template <bool C, bool A, int FC>
 static int launch_front_fc(rts_pipeline *p, float *P, int W, int H, int D, int vec, hipStream_t s) {
     const int waves = div_up(D, 64);
-    const size_t shm = (size_t)W * ((C ? 8 : 0) + (A ? 4 : 0)) + sizeof(float) * (65 + p->nA) +
-                       sizeof(float) * waves * 64 * (FC + 1);
+    const size_t shm = (size_t)W * ((C ? 8 : 0) + (A ? 4 : 0)) + sizeof(float) * (65 + p->nA) + 16 +
+                       sizeof(float) * waves * 64 * FC;
     RTS_CHECK(shm <= 160 * 1024, "front-end LDS %zu too large", shm);
     if (shm > 65536) {   // the kernel's dynamic LDS above the default 64 KiB
         static bool raised = false;
