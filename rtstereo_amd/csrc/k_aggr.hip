@@ -673,12 +673,14 @@ __device__ __forceinline__ float div_count(float a, uint32_t c) {
 // The sweep for arms of at most CV_LMAX (seq/CrossBasedAggregation.cpp:127-169,
 // bit-identical).  A block of NW waves owns 64 columns and SEG output rows of
 // one d plane and walks rows [s0-17, s0+SEG+17) in steps of CH; every wave
-// forms CH/NW rows of segment sums per step and then emits every output row
+// forms CH/NW rows of segment sums per step and then computes the output rows
 // whose support is complete, NW rows apart.
 //  - Segment sums: a row's 99 prefix values (columns x0-18 .. x0+80) are
 //    loaded coalesced, two registers per lane, one step ahead; each lane
 //    picks its segment's two ends with ds_bpermute, so every segment sum is
-//    formed once (no tile halo).
+//    formed once (no tile halo).  Loads are buffer loads: the plane and map
+//    bases sit in scalar registers, the row as a scalar offset, the column as
+//    a fixed per-lane offset.
 //  - The ring keeps R = CH + 34 live rows and repeats its first 34 rows past
 //    its end, so the rows y-17 .. y+17 of any output sit at consecutive
 //    positions and every read of a vertical sum is an immediate offset from
@@ -687,8 +689,13 @@ __device__ __forceinline__ float div_count(float a, uint32_t c) {
 //    of the plain 32-bit sum of the entries over a support are its count (at
 //    most 35 rows of at most 35 columns), so no field is unpacked per row.
 //  - A support of n rows is summed as n % 4 single rows and then n / 4
-//    groups of four, in row order from +0.0 (the reference's order), with
-//    exec-masked guards instead of loops; the quotient uses div_count.
+//    groups of four, in row order from +0.0 (the reference's order); the
+//    quotient uses div_count.
+//  - A step's outputs (at most CH/NW rows per wave) stay in registers and are
+//    stored after the next step's segment sums, before that step's loads:
+//    loads and stores share one completion counter, so a wait for a step's
+//    loads then never waits for the stores of the step before.  The rows a
+//    step past the image bottom adds (up to 17) are computed after the walk.
 // Eight waves share one ring, which keeps six waves per SIMD resident at
 // 51 KB of LDS per block.
 template <int CH, int NW>
@@ -729,27 +736,31 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
     // then lies in lo for lanes below 47 and in hi (source lane jl mod 64)
     // for the others; a right end jr in [lane+18, lane+35] lies in lo for
     // lanes below 29 and in hi for the others, so the register is a fixed
-    // choice per lane.
+    // choice per lane.  Segment ends are always columns inside the image, so
+    // a lane whose column lies outside it loads any in-image column instead.
     const int c0 = cs + lane, c1 = cs + lane + (lane < 47 ? 64 : 0);
-    const bool ok0 = c0 >= 0 && c0 < W, ok1 = (lane < CV_COLS - 64 || lane >= 47) && c1 >= 0 && c1 < W;
+    const uint32_t oc0 = (uint32_t)min(max(c0, 0), W - 1) * 4u, oc1 = (uint32_t)min(max(c1, 0), W - 1) * 4u;
     // blocks whose 99 columns all lie inside the image need no clamping
     const bool interior = cs >= 0 && cs + CV_COLS <= W;
-    const float *Pb = P + base;
-    // uniform row bases with 32-bit per-lane byte offsets (scalar-base addressing)
     const uint32_t oxc = (uint32_t)xc * 4u, otx = (uint32_t)tx * 4u;
-    const uint32_t oc0 = (uint32_t)c0 * 4u, oc1 = (uint32_t)c1 * 4u;
     const bool hi_l = lane >= 47, hi_r = lane >= 29;
+    // one plane is at most 2^32 bytes (W*H <= 2^30), so every offset fits 32 bits
+    const uint64_t pb64 = (uint64_t)W * (uint64_t)H * 4u;
+    const int pbytes = (int)(pb64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)pb64);
+    const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void *)(P + base), 0, pbytes, BUF_DWORD3);
+    const auto rsR = __builtin_amdgcn_make_buffer_rsrc((void *)aref, 0, pbytes, BUF_DWORD3);
+    const auto rsT = __builtin_amdgcn_make_buffer_rsrc((void *)atar, 0, pbytes, BUF_DWORD3);
     // A step's loads are issued one step before its segment sums are formed.
     float p0[KR], p1[KR];
     uint32_t ar[KR], at[KR];
     auto load_rows = [&](int r) {
 #pragma unroll
         for (int k = 0; k < KR; k++) {
-            const size_t ro = (size_t)min(r + w + NW * k, rhi - 1) * W;
-            p0[k] = ok0 ? at_byte(Pb + ro, oc0) : 0.0f;
-            p1[k] = ok1 ? at_byte(Pb + ro, oc1) : 0.0f;
-            ar[k] = at_byte(aref + ro, oxc);
-            at[k] = at_byte(atar + ro, otx);
+            const uint32_t ro = (uint32_t)min(r + w + NW * k, rhi - 1) * (uint32_t)W * 4u;
+            p0[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsP, oc0, ro, 0));
+            p1[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsP, oc1, ro, 0));
+            ar[k] = __builtin_amdgcn_raw_buffer_load_b32(rsR, oxc, ro, 0);
+            at[k] = __builtin_amdgcn_raw_buffer_load_b32(rsT, otx, ro, 0);
         }
     };
     int out_next = s0;
@@ -786,60 +797,98 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
             }
         }
     };
+    // The output whose row y-17 sits at ring position bm; wd is the entry
+    // word of row y itself.
+    auto out_row = [&](int bm, uint32_t wd) -> float {
+        const int t = (wd >> 16) & 0xFF, b = wd >> 24;
+        const int n = t + b + 1, lead = n & 3, q = n >> 2;
+        const float2 *s = &ring[bm][lane] + (CV_LMAX - t) * CV_TX;
+        float sum = 0.0f;
+        uint32_t cw = 0;
+        if (lead) {
+            // the lead rows in one round trip; a row past the lead adds +0.0,
+            // which leaves the (never -0.0) sum unchanged
+            const float2 a0 = s[0], a1 = s[CV_TX], a2 = s[2 * CV_TX];
+            const bool u1 = lead > 1, u2 = lead > 2;
+            sum = ((sum + a0.x) + (u1 ? a1.x : 0.0f)) + (u2 ? a2.x : 0.0f);
+            cw = __float_as_uint(a0.y) + (u1 ? __float_as_uint(a1.y) : 0u) + (u2 ? __float_as_uint(a2.y) : 0u);
+        }
+        s += lead * CV_TX;
+        if (q >= 1) {
+            const float2 e0 = s[0], e1 = s[CV_TX], e2 = s[2 * CV_TX], e3 = s[3 * CV_TX];
+            sum = (((sum + e0.x) + e1.x) + e2.x) + e3.x;
+            cw += __float_as_uint(e0.y) + __float_as_uint(e1.y) + __float_as_uint(e2.y) + __float_as_uint(e3.y);
+        }
+        // supports of 8 rows or more (rare): further groups of four
+#pragma unroll 1
+        for (int gi = 1; gi < q; gi++) {
+            const float2 *gp = s + 4 * gi * CV_TX;
+            const float2 e0 = gp[0], e1 = gp[CV_TX], e2 = gp[2 * CV_TX], e3 = gp[3 * CV_TX];
+            sum = (((sum + e0.x) + e1.x) + e2.x) + e3.x;
+            cw += __float_as_uint(e0.y) + __float_as_uint(e1.y) + __float_as_uint(e2.y) + __float_as_uint(e3.y);
+        }
+        return div_count(sum, cw & 0xFFFFu);
+    };
+    auto store_row = [&](int y, float v) {
+        float *orow = out + base + (size_t)y * W;
+        *(float *)((char *)orow + (uint32_t)x * 4u) = v;
+    };
+    float res[KR];                 // this wave's outputs of the last step:
+    int res_y = 0, res_end = 0;    // rows res_y + NW*i below res_end
     for (int r = rlo; r < rhi; r += CH) {
         if (interior)
             produce(std::true_type{}, r);
         else
             produce(std::false_type{}, r);
         __syncthreads();
+        if (colok) {
+#pragma unroll
+            for (int i = 0; i < KR; i++)
+                if (res_y + NW * i < res_end) store_row(res_y + NW * i, res[i]);
+        }
         const int rnext = r + CH;
         if (rnext < rhi) load_rows(rnext);
         const int avail = min(rnext, rhi);
         const int out_end = avail >= rhi ? s1 : min(s1, avail - CV_LMAX);
+        // at most CH rows a step; the rest of the last step's rows follow the walk
+        const int oe = min(out_end, out_next + CH);
+        res_y = out_next + w;
+        res_end = oe;
         if (colok) {
             // ring position of row y-17 for the wave's first output row (rows
             // below rlo never enter a support), then advanced with y; each
-            // output row's entry word is read one iteration ahead
+            // output row's entry word is read one row ahead
             int bm = (out_next + w - CV_LMAX - rlo + R) % R;
             uint32_t word = __float_as_uint(ring[bm + CV_LMAX][lane].y);
-            for (int y = out_next + w; y < out_end; y += NW) {
-                const float2 *col = &ring[bm][lane];
-                bm += NW;
-                if (bm >= R) bm -= R;
-                const uint32_t next = __float_as_uint(ring[bm + CV_LMAX][lane].y);
-                const int t = (word >> 16) & 0xFF, b = word >> 24;
-                const int n = t + b + 1, lead = n & 3, q = n >> 2;
-                const float2 *s = col + (CV_LMAX - t) * CV_TX;
-                float sum = 0.0f;
-                uint32_t cw = 0;
-                if (lead) {
-                    // the lead rows in one round trip; a row past the lead adds
-                    // +0.0, which leaves the (never -0.0) sum unchanged
-                    const float2 a0 = s[0], a1 = s[CV_TX], a2 = s[2 * CV_TX];
-                    const bool u1 = lead > 1, u2 = lead > 2;
-                    sum = ((sum + a0.x) + (u1 ? a1.x : 0.0f)) + (u2 ? a2.x : 0.0f);
-                    cw = __float_as_uint(a0.y) + (u1 ? __float_as_uint(a1.y) : 0u) +
-                         (u2 ? __float_as_uint(a2.y) : 0u);
-                }
-                s += lead * CV_TX;
 #pragma unroll
-                for (int gi = 0; gi < (2 * CV_LMAX + 1) / 4; gi++) {
-                    if (gi >= q) break;
-                    const float2 e0 = s[(4 * gi) * CV_TX], e1 = s[(4 * gi + 1) * CV_TX];
-                    const float2 e2 = s[(4 * gi + 2) * CV_TX], e3 = s[(4 * gi + 3) * CV_TX];
-                    sum = (((sum + e0.x) + e1.x) + e2.x) + e3.x;
-                    cw += __float_as_uint(e0.y) + __float_as_uint(e1.y) + __float_as_uint(e2.y) +
-                          __float_as_uint(e3.y);
+            for (int i = 0; i < KR; i++) {
+                if (res_y + NW * i < oe) {
+                    int bn = bm + NW;
+                    if (bn >= R) bn -= R;
+                    const uint32_t next = __float_as_uint(ring[bn + CV_LMAX][lane].y);
+                    res[i] = out_row(bm, word);
+                    bm = bn;
+                    word = next;
                 }
-                word = next;
-                float *orow = out + base + (size_t)y * W;
-                *(float *)((char *)orow + (uint32_t)x * 4u) = div_count(sum, cw & 0xFFFFu);
             }
         }
-        out_next = max(out_next, out_end);
+        out_next = max(out_next, oe);
         pr += CH;
         if (pr >= R) pr -= R;
         __syncthreads();
+    }
+    if (colok) {
+#pragma unroll
+        for (int i = 0; i < KR; i++)
+            if (res_y + NW * i < res_end) store_row(res_y + NW * i, res[i]);
+        // rows past the last step's CH outputs (a segment at the image
+        // bottom); every row of their supports is still in the ring
+        int bm = (out_next + w - CV_LMAX - rlo + R) % R;
+        for (int y = out_next + w; y < s1; y += NW) {
+            store_row(y, out_row(bm, __float_as_uint(ring[bm + CV_LMAX][lane].y)));
+            bm += NW;
+            if (bm >= R) bm -= R;
+        }
     }
 }
 

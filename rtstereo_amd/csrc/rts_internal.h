@@ -54,6 +54,10 @@ __device__ __forceinline__ int wrapn(int v, int n) {
 // rounded once to float is, because 53 >= 2*24 + 2 bits.
 __device__ __forceinline__ float sqrt_rn(float x) { return (float)__builtin_sqrt((double)x); }
 
+// Word 3 of a raw buffer resource on CDNA (gfx9 family): 32-bit data format,
+// no swizzle, no stride, so an access is base + soffset + voffset.
+constexpr int BUF_DWORD3 = 0x00020000;
+
 // XCD-aware block order.  MI355X hands consecutive workgroups of a launch
 // to its 8 XCDs round-robin, and each XCD has its own L2.  Given the linear
 // workgroup id, return a work index such that XCD k processes one
