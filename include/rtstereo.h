@@ -319,6 +319,13 @@ rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint
  * waits for the last run and returns the stage durations in ms. */
 rts_status rts_pipeline_enable_timing(rts_pipeline *p, int on);
 rts_status rts_pipeline_stage_ms(rts_pipeline *p, float *ms, const char **names, int max, int *n);
+/* Graph replay: when enabled, the second run with the same image, output
+ * and stream pointers captures the run's launch sequence into a HIP graph
+ * and every later run with those pointers replays it (new contents in the
+ * same buffers are read as usual).  Runs with other pointers, on the null
+ * stream or with timing enabled launch their kernels one by one.  No
+ * reference counterpart (the reference runs on the host). */
+rts_status rts_pipeline_enable_graphs(rts_pipeline *p, int on);
 /* Device pointer of the final aggregated float volume of the last run
  * (disparity-major), for inspection. */
 const float *rts_pipeline_volume(rts_pipeline *p);

@@ -97,6 +97,7 @@ _SIGS = {
     "rts_pipeline_run": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
     "rts_pipeline_volume": [vp],
     "rts_pipeline_enable_timing": [vp, i32],
+    "rts_pipeline_enable_graphs": [vp, i32],
     "rts_pipeline_stage_ms": [vp, C.POINTER(f32), C.POINTER(cp), i32, C.POINTER(i32)],
     "rts_rgb_to_gray": [vp, vp, i32, i32, vp],
     "rts_rgb_to_lab": [vp, vp, i32, i32, vp],

@@ -245,6 +245,16 @@ struct rts_pipeline {
     int nA = 0;
     int census_words = 1;
     bool fused_front = false;
+    // HIP graph of the launch sequence, replayed for runs with the buffers
+    // and stream it was captured with
+    struct RunKey {
+        const void *ptr[10] = {};
+        bool operator==(const RunKey &o) const { return memcmp(ptr, o.ptr, sizeof ptr) == 0; }
+    };
+    bool graphs = false;
+    RunKey gseen, gkey;
+    hipGraphExec_t gexec = nullptr;
+    float *gfinal = nullptr;
     // per-stage timing (HIP events recorded on the run's stream)
     bool timing = false;
     hipEvent_t ev[8] = {};
@@ -415,6 +425,7 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
 
 rts_status rts_pipeline_destroy(rts_pipeline *p) {
     if (!p) return RTS_OK;
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     for (int v = 0; v < 2; v++) {
         (void)hipFree(p->gray[v]);
         (void)hipFree(p->census[v]);
@@ -494,11 +505,11 @@ static int launch_front(rts_pipeline *p, float *P, int W, int H, int D, int vec,
     return launch_front_fc<C, A, 32>(p, P, W, H, D, vec, s);
 }
 
-extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
-                                       const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
-                                       const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask,
-                                       void *stream) {
-    RTS_CHECK(p != nullptr, "null pipeline");
+// The frame's launch sequence on stream `stream` (eager, or recorded by a
+// stream capture).
+static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
+                                  const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
+                                  const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask, void *stream) {
     const rts_pipeline_desc &d = p->d;
     const int W = d.width, H = d.height, D = d.disp;
     hipStream_t s = S(stream);
@@ -600,6 +611,64 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
     if (int e = rts_wta(cur, dl, d.lr_check ? dr : nullptr, d.lr_check ? mask : nullptr, W, H, D, flags, d.lr_tol, s))
         return e;
     RTS_MARK(p, "wta", s);
+    return RTS_OK;
+}
+
+extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
+                                       const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
+                                       const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask,
+                                       void *stream) {
+    RTS_CHECK(p != nullptr, "null pipeline");
+    if (!p->graphs || p->timing || stream == nullptr)
+        return pipeline_launch(p, left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,
+                               stream);
+    const rts_pipeline::RunKey key{{left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,
+                                    stream}};
+    hipStream_t s = S(stream);
+    if (p->gexec && key == p->gkey) {
+        RTS_HIP(hipGraphLaunch(p->gexec, s));
+        p->final_vol = p->gfinal;
+        return RTS_OK;
+    }
+    if (!(key == p->gseen)) {   // first run with these buffers: eager
+        p->gseen = key;
+        return pipeline_launch(p, left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,
+                               stream);
+    }
+    // second run with the same buffers: capture the launch sequence once
+    if (p->gexec) {
+        (void)hipGraphExecDestroy(p->gexec);
+        p->gexec = nullptr;
+    }
+    RTS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int e = pipeline_launch(p, left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,
+                                  stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    if (e != RTS_OK || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return e != RTS_OK ? e : fail(RTS_ERR_HIP, "pipeline capture: %s", hipGetErrorString(ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        p->gexec = nullptr;
+        return fail(RTS_ERR_HIP, "pipeline graph instantiate: %s", hipGetErrorString(ie));
+    }
+    p->gkey = key;
+    p->gfinal = p->final_vol;
+    RTS_HIP(hipGraphLaunch(p->gexec, s));
+    return RTS_OK;
+}
+
+extern "C" rts_status rts_pipeline_enable_graphs(rts_pipeline *p, int on) {
+    RTS_CHECK(p != nullptr, "null pipeline");
+    p->graphs = on != 0;
+    if (!p->graphs && p->gexec) {
+        RTS_HIP(hipGraphExecDestroy(p->gexec));
+        p->gexec = nullptr;
+    }
+    p->gseen = p->gkey = rts_pipeline::RunKey{};
     return RTS_OK;
 }
 

@@ -92,6 +92,11 @@ class Pipeline:
     def enable_timing(self, on: bool = True) -> None:
         call("rts_pipeline_enable_timing", self.handle, int(on))
 
+    def enable_graphs(self, on: bool = True) -> None:
+        """Replay the run's launch sequence as a HIP graph for repeated runs
+        with the same buffers and stream (captured on the second such run)."""
+        call("rts_pipeline_enable_graphs", self.handle, int(on))
+
     def stage_ms(self) -> dict:
         """Stage durations (ms) of the last run, measured with HIP events on
         the run's stream; empty unless timing is enabled."""
