@@ -736,9 +736,11 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
     // then lies in lo for lanes below 47 and in hi (source lane jl mod 64)
     // for the others; a right end jr in [lane+18, lane+35] lies in lo for
     // lanes below 29 and in hi for the others, so the register is a fixed
-    // choice per lane.  Segment ends are always columns inside the image, so
-    // a lane whose column lies outside it loads any in-image column instead.
-    const int c0 = cs + lane, c1 = cs + lane + (lane < 47 ? 64 : 0);
+    // choice per lane.  hi on lanes 35 .. 46 is never read: those lanes load
+    // their lo column again, which keeps every load inside the window.
+    // Segment ends are always columns inside the image, so a lane whose
+    // column lies outside it loads any in-image column instead.
+    const int c0 = cs + lane, c1 = cs + lane + (lane < CV_COLS - 64 ? 64 : 0);
     const uint32_t oc0 = (uint32_t)min(max(c0, 0), W - 1) * 4u, oc1 = (uint32_t)min(max(c1, 0), W - 1) * 4u;
     // blocks whose 99 columns all lie inside the image need no clamping
     const bool interior = cs >= 0 && cs + CV_COLS <= W;
