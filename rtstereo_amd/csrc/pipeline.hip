@@ -619,7 +619,9 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
                                        const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask,
                                        void *stream) {
     RTS_CHECK(p != nullptr, "null pipeline");
-    if (!p->graphs || p->timing || stream == nullptr)
+    // the literal WTA carry allocates its scratch on the stream per run, so
+    // those pipelines are never captured
+    if (!p->graphs || p->timing || stream == nullptr || p->d.wta_literal)
         return pipeline_launch(p, left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,
                                stream);
     const rts_pipeline::RunKey key{{left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,

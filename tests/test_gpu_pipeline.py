@@ -139,7 +139,10 @@ def test_census_cross_1080p_properties(exact):
 @pytest.mark.parametrize("name,W,H,D,over", [
     ("census_cross", 96, 64, 24, {}),
     ("census_cross", 150, 70, 40, {"cross_iters": 2}),
+    ("census_cross", 96, 64, 24, {"wta_literal": 1}),
     ("plain", 96, 64, 16, {}),
+    ("aw", 80, 48, 16, {"aw_r": 5}),
+    ("rank_log_ncc", 96, 64, 24, {}),
 ])
 def test_pipeline_graph_replay(name, W, H, D, over):
     """Graph replay: runs with the same buffers and stream are captured once
