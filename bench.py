@@ -51,6 +51,19 @@ STAGE_TRAFFIC = {"cost": (30386.0 * 2 + 1555200.0) * 1024, "aggr": (1398682.6 * 
                  "wta": (792811.7 * 2 + 4050.0) * 1024}
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo), for the baseline record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(target_s: float) -> dict:
     """The reference's own seq/ stage functions (compiled from its sources
     into oracle/_ref) on a full-width strip of the same frame, one core;
@@ -92,7 +105,8 @@ def cpu_baseline(target_s: float) -> dict:
               "sample": f"census+AD combine, cross L17/tao20, WTA on a {W}x{rows} strip (rows 500..{500 + rows - 1}) "
                         f"of the 1080p frame, D={D}: {pd} pd in {t:.2f} s, "
                         f"{'seq/ sources compiled with g++ -O2' if kind == 'reference' else 'C oracle'}, single thread",
-              "frame_seconds_extrapolated": W * H * D / (pd / t)}
+              "frame_seconds_extrapolated": W * H * D / (pd / t), "cpu": cpu_model(),
+              "host_cores": os.cpu_count()}
     # All host cores (at most 16, the share of one GPU): independent
     # 48-row strips of the same frame, one per thread (the stage functions
     # are reentrant and the foreign calls release the interpreter lock).
@@ -106,7 +120,8 @@ def cpu_baseline(target_s: float) -> dict:
     tw = time.perf_counter() - t0
     pdn = W * per * D * n
     parallel = {"value": pdn / tw, "unit": "pd/s", "cores": n, "kind": kind,
-                "sample": f"{n} threads, each a {W}x{per} strip of the same pipeline, {pdn} pd in {tw:.2f} s wall"}
+                "sample": f"{n} threads, each a {W}x{per} strip of the same pipeline, {pdn} pd in {tw:.2f} s wall",
+                "cpu": cpu_model()}
     return single, parallel
 
 
