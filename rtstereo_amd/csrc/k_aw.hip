@@ -10,13 +10,14 @@
 // writes both views' weights of the band's centre pixels to a workspace,
 // laid out [row][dy][tap][x] so that a tap row of consecutive pixels is
 // contiguous.  The aggregation kernel then works on tiles of 64 pixels x 2
-// rows x 64 disparities: lane = disparity, wave = 8 consecutive pixels.
-// Per window row it stages in LDS the 64 cost rows of its disparities and
-// the target weights of the 127 centres x-d the tile touches; the reference
-// weights of a wave's 8 pixels are uniform across its lanes and come in
-// through scalar loads.  Each lane then accumulates 2 rows x 8 pixels of
-// its disparity, reading the target weight from LDS and the cost from a
-// register window that slides along the tap row.
+// rows x 64 disparities: lane = disparity, wave = 8 pixels (FP32: 8
+// consecutive pixels; FP16: the even or the odd pixels of a group of 16).
+// Per window row it stages in LDS the 64 cost rows of its disparities, the
+// target weights of the 127 centres x-d the tile touches and the reference
+// weights of the tile's pixels; a wave's reference weights are the same
+// address on all its lanes (broadcast LDS reads).  Each lane then
+// accumulates 2 rows x 8 pixels of its disparity, reading the weights from
+// LDS and the cost from a register window that slides along the tap row.
 //
 //   FP32: product, sum, fused multiply-add per tap (3 VALU);
 //   FP16: weights stored as half pairs of neighbouring taps, one packed
