@@ -223,14 +223,17 @@ __global__ void k_disp_refine_literal(uint8_t *__restrict__ disp, const uint32_t
 }
 
 // Rows are cut into chunks, one block each.  One wave per block and 8 pixels per lane (512-pixel row chunks): a 1080p
-// frame gives 4320 blocks, all resident at once, spread evenly over the CUs,
-// with 2*4*8 loads in flight per lane.
-constexpr int WTA_T = 64, WTA_DU = 4;
+// frame gives 4320 blocks, all resident at once, spread evenly over the CUs.
+// With 8 pixels per lane one disparity at a time (2*8 loads in flight per
+// lane) measured fastest (0.27 against 0.29 ms with four at a time at
+// 1080p/192); narrower rows, with fewer pixels per lane, batch four.
+constexpr int WTA_T = 64;
 
 template <int PPT, bool MAXIMIZE, bool WANT_R>
 static void wta_go(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *beat, int W, int H, int D, hipStream_t s) {
-    k_wta<PPT, MAXIMIZE, WANT_R, WTA_DU, WTA_T><<<dim3(div_up(W, WTA_T * PPT), H), WTA_T, 0, s>>>(vol, dl, dr, beat, W,
-                                                                                                 H, D);
+    constexpr int DU = PPT >= 8 ? 1 : 4;
+    k_wta<PPT, MAXIMIZE, WANT_R, DU, WTA_T><<<dim3(div_up(W, WTA_T * PPT), H), WTA_T, 0, s>>>(vol, dl, dr, beat, W, H,
+                                                                                             D);
 }
 
 template <bool MAXIMIZE, bool WANT_R>
