@@ -692,10 +692,11 @@ __device__ __forceinline__ float div_count(float a, uint32_t c) {
 //    groups of four, in row order from +0.0 (the reference's order); the
 //    quotient uses div_count.
 //  - A step's outputs (at most CH/NW rows per wave) stay in registers and are
-//    stored after the next step's segment sums, before that step's loads:
-//    loads and stores share one completion counter, so a wait for a step's
-//    loads then never waits for the stores of the step before.  The rows a
-//    step past the image bottom adds (up to 17) are computed after the walk.
+//    stored (non-temporal) after the next step's segment sums, before that
+//    step's loads: loads and stores share one completion counter, so a wait
+//    for a step's loads then never waits for the stores of the step before.
+//    The rows a step past the image bottom adds (up to 17) are computed
+//    after the walk.
 // Eight waves share one ring, which keeps six waves per SIMD resident at
 // 51 KB of LDS per block.
 template <int CH, int NW>
@@ -831,9 +832,12 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
         }
         return div_count(sum, cw & 0xFFFFu);
     };
+    // non-temporal stores: the aggregated volume streams out to the next
+    // kernel (measured 1.03 -> 1.00 ms here and 0.30 -> 0.28 ms for the WTA
+    // that reads it, at 1080p/192)
     auto store_row = [&](int y, float v) {
         float *orow = out + base + (size_t)y * W;
-        *(float *)((char *)orow + (uint32_t)x * 4u) = v;
+        __builtin_nontemporal_store(v, (float *)((char *)orow + (uint32_t)x * 4u));
     };
     float res[KR];                 // this wave's outputs of the last step:
     int res_y = 0, res_end = 0;    // rows res_y + NW*i below res_end

@@ -44,6 +44,8 @@ __device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
     return *(const T *)((const char *)base + off);
 }
 
+typedef float f4v __attribute__((ext_vector_type(4)));
+
 // A value the compiler cannot see through (keeps a uniform offset in a VGPR).
 __device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
     asm volatile("" : "+v"(v));
@@ -155,8 +157,11 @@ __global__ void __launch_bounds__(256) k_front_hscan(const uint64_t *__restrict_
             auto rd = [&](int it) {
                 return *reinterpret_cast<const float4 *>(stage_at(it * RPI + lane / LPR, lane % LPR));
             };
+            // non-temporal: the prefix volume streams out (it is read once,
+            // by the next kernel), measured 0.57 -> 0.45 ms at 1080p/192
             auto wr = [&](int it, const float4 &v, bool guard) {
-                if (!guard || dw + it * RPI + lane / LPR < D) *reinterpret_cast<float4 *>(Pw + it * pstep) = v;
+                if (!guard || dw + it * RPI + lane / LPR < D)
+                    __builtin_nontemporal_store(f4v{v.x, v.y, v.z, v.w}, reinterpret_cast<f4v *>(Pw + it * pstep));
             };
             const bool guard = dw + 64 > D;
 #pragma unroll
