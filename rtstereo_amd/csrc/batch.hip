@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
                 if (y >= H) break;
                 float acc = 0.0f;
                 for (int dy = 0; dy < e; dy++) acc += hs[ly + dy][tx];
-                bp.out[base + (size_t)y * W + x] = __fdiv_rn(acc, area);
+                stream_store(&bp.out[base + (size_t)y * W + x], __fdiv_rn(acc, area));
             }
     }
 }

@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) k_box_f32(const float *__restrict__ in, f
         float acc = 0.0f;
         for (int dy = 0; dy <= 2 * r; dy++)
             for (int dx = 0; dx <= 2 * r; dx++) acc += tile[ly + dy][threadIdx.x + dx];
-        out[base + (size_t)y * W + x] = __fdiv_rn(acc, area);
+        stream_store(&out[base + (size_t)y * W + x], __fdiv_rn(acc, area));
     }
 }
 

@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
         for (int i = tid; i < 64 * TXP; i += NTHR) {
             const int dd = i / TXP, px = i % TXP;
             const int d = d0 + dd, x = x0 + px;
-            if (d < D && x < W) out[(size_t)d * pl + (size_t)y * W + x] = tile[dd * 65 + px];
+            if (d < D && x < W) stream_store(&out[(size_t)d * pl + (size_t)y * W + x], tile[dd * 65 + px]);
         }
     }
 }

@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restric
                     const float t1 = 1.0f - __builtin_amdgcn_exp2f(cr * k1);
                     const float t2 = 1.0f - __builtin_amdgcn_exp2f(cl * k2);
                     const float t3 = 1.0f - __builtin_amdgcn_exp2f((1.0f - sn) * k3);
-                    vol[d * pl + o + x] = (t1 + t2) + t3;
+                    stream_store(&vol[d * pl + o + x], (t1 + t2) + t3);
                 }
             }
         }

@@ -54,6 +54,10 @@ __device__ __forceinline__ int wrapn(int v, int n) {
 // rounded once to float is, because 53 >= 2*24 + 2 bits.
 __device__ __forceinline__ float sqrt_rn(float x) { return (float)__builtin_sqrt((double)x); }
 
+// Store of a volume element that the next kernel reads once: non-temporal,
+// so streamed volumes do not displace reusable lines from the caches.
+__device__ __forceinline__ void stream_store(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
 // Word 3 of a raw buffer resource on CDNA (gfx9 family): 32-bit data format,
 // no swizzle, no stride, so an access is base + soffset + voffset.
 constexpr int BUF_DWORD3 = 0x00020000;
