@@ -399,6 +399,11 @@ typedef struct rts_server_stats {
 } rts_server_stats;
 
 typedef struct rts_server rts_server;
+/* Each slot runs on a HIP stream of its own.  HIP spreads streams
+ * round-robin over GPU_MAX_HW_QUEUES hardware queues (4 by default): a
+ * serving process should start with GPU_MAX_HW_QUEUES >= slots + 1 (at most
+ * 32), or frames of streams that share a queue run in queue order behind
+ * each other's kernels (python -m rtstereo_amd.serve sets 2 x slots). */
 rts_status rts_server_create(size_t budget_bytes, int slots, rts_server **out);
 rts_status rts_server_destroy(rts_server *s);     /* cancels queued requests, waits for running ones */
 /* RTS_ERR_BUDGET if the request alone exceeds the budget. */

@@ -53,6 +53,14 @@ struct Request {
     bool done = false, started = false;
 };
 
+// Work for the allocator thread: pipelines to free (already out of the
+// resident set and the budget) and the request whose pipeline to create
+// (its footprint already reserved).
+struct AllocJob {
+    std::vector<std::unique_ptr<Resident>> victims;
+    std::shared_ptr<Request> r;
+};
+
 size_t io_bytes(const rts_pipeline_desc &d) {
     const size_t P = (size_t)d.width * d.height;
     return 2 * 3 * P + 2 * 3 * P + 3 * P;   // rgb, lab, dl/dr/mask
@@ -75,6 +83,13 @@ struct rts_server {
     std::vector<Resident *> assigned_res;
     std::vector<std::thread> workers;
     std::thread dispatcher;
+    // Pipeline creation and release run on their own thread: hipFree waits
+    // for the device's outstanding work, and the dispatcher must keep
+    // feeding resident camera pipelines meanwhile.
+    std::deque<AllocJob> jobs;
+    int preparing = 0;   // jobs queued or running
+    std::condition_variable cv_alloc;
+    std::thread allocator;
     uint64_t completed = 0;
 };
 
@@ -114,35 +129,41 @@ size_t idle_bytes(const rts_server *s) {
     return n;
 }
 
-// Frees idle pipelines until `need` fits: pool pipelines first, then idle
-// camera pipelines (which are rebuilt on the stream's next frame), least
-// recently used first within each group.
-void evict_for(rts_server *s, size_t need) {
+// Takes idle pipelines out of the resident set until `need` fits: pool
+// pipelines first, then idle camera pipelines (which are rebuilt on the
+// stream's next frame), least recently used first within each group.  The
+// caller frees them (free_resident) without holding the lock.
+void take_victims(rts_server *s, size_t need, std::vector<std::unique_ptr<Resident>> &out) {
     while (s->in_use + need > s->budget) {
-        Resident *victim = nullptr;
-        for (auto &res : s->residents) {
-            if (res->busy) continue;
-            if (!victim) {
-                victim = res.get();
+        auto victim = s->residents.end();
+        for (auto it = s->residents.begin(); it != s->residents.end(); ++it) {
+            const Resident &res = **it;
+            if (res.busy) continue;
+            if (victim == s->residents.end()) {
+                victim = it;
                 continue;
             }
-            const bool pool = res->stream_id < 0, vpool = victim->stream_id < 0;
-            if ((pool && !vpool) || (pool == vpool && res->last_use < victim->last_use)) victim = res.get();
+            const bool pool = res.stream_id < 0, vpool = (*victim)->stream_id < 0;
+            if ((pool && !vpool) || (pool == vpool && res.last_use < (*victim)->last_use)) victim = it;
         }
-        if (!victim) return;
-        s->in_use -= victim->footprint;
-        rts_pipeline_destroy(victim->pipe);
-        (void)hipFree(victim->dev);
-        (void)hipHostFree(victim->host);
-        s->residents.erase(std::find_if(s->residents.begin(), s->residents.end(),
-                                        [&](const std::unique_ptr<Resident> &p) { return p.get() == victim; }));
+        if (victim == s->residents.end()) return;
+        s->in_use -= (*victim)->footprint;
+        out.push_back(std::move(*victim));
+        s->residents.erase(victim);
     }
 }
 
-int make_resident(rts_server *s, const rts_pipeline_desc &d, int stream_id, Resident **out) {
-    if (stream_id >= 0)   // a camera that changed configuration: its old pipeline joins the pool
-        for (auto &res : s->residents)
-            if (res->stream_id == stream_id) res->stream_id = -1;
+void free_resident(Resident &res) {
+    rts_pipeline_destroy(res.pipe);
+    (void)hipFree(res.dev);
+    (void)hipHostFree(res.host);
+    res.pipe = nullptr;
+    res.dev = res.host = nullptr;
+}
+
+// A new pipeline with its device and pinned staging buffers (no server
+// state touched).
+int create_resident(const rts_pipeline_desc &d, int stream_id, std::unique_ptr<Resident> &out) {
     std::unique_ptr<Resident> res(new Resident());
     res->desc = d;
     res->stream_id = stream_id;
@@ -151,14 +172,51 @@ int make_resident(rts_server *s, const rts_pipeline_desc &d, int stream_id, Resi
     if (hipMalloc((void **)&res->dev, res->dev_bytes) != hipSuccess ||
         hipHostMalloc((void **)&res->host, res->dev_bytes, hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
-        rts_pipeline_destroy(res->pipe);
-        (void)hipFree(res->dev);
+        free_resident(*res);
         return rts::fail(RTS_ERR_NOMEM, "server: staging allocation of %zu bytes failed", res->dev_bytes);
     }
     res->footprint = pipeline_footprint(d);
-    *out = res.get();
-    s->residents.push_back(std::move(res));
+    out = std::move(res);
     return RTS_OK;
+}
+
+void allocator_main(rts_server *s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_alloc.wait(lk, [&] { return s->stop || !s->jobs.empty(); });
+        if (s->jobs.empty()) break;   // stop with nothing left to do
+        AllocJob job = std::move(s->jobs.front());
+        s->jobs.pop_front();
+        const bool stopping = s->stop;
+        lk.unlock();
+        for (auto &v : job.victims) free_resident(*v);
+        job.victims.clear();
+        std::unique_ptr<Resident> res;
+        const int e = stopping ? RTS_ERR_CANCELLED
+                               : create_resident(job.r->req.pipe, job.r->req.stream_id, res);
+        lk.lock();
+        s->preparing--;
+        std::shared_ptr<Request> r = job.r;
+        if (e != RTS_OK || s->stop) {
+            s->in_use -= r->footprint;
+            if (res) {
+                lk.unlock();
+                free_resident(*res);
+                lk.lock();
+            }
+            r->st.status = e != RTS_OK ? e : RTS_ERR_CANCELLED;
+            r->done = true;
+            s->cv_done.notify_all();
+        } else {
+            // a camera that changed configuration: its old pipeline joins the pool
+            if (res->stream_id >= 0)
+                for (auto &old : s->residents)
+                    if (old->stream_id == res->stream_id) old->stream_id = -1;
+            s->residents.push_back(std::move(res));
+            s->queue.push_front(r);   // next in line for its new pipeline
+        }
+        s->cv_work.notify_all();
+    }
 }
 
 // One request on one slot: stage inputs, run, stage outputs.
@@ -223,7 +281,11 @@ void worker_main(rts_server *s, int slot) {
 // once idle pipelines are evicted.  Once a request of a pass is waiting for
 // memory, the requests behind it in that pass may still start on resident
 // pipelines but may not allocate, so a large request is not starved by a
-// stream of smaller ones.
+// stream of smaller ones.  A request that needs a new pipeline reserves its
+// footprint and leaves the queue; the allocator thread frees the evicted
+// pipelines, builds the new one and puts the request back at the head of
+// the queue (one such request at a time), so that frames of resident camera
+// pipelines keep starting while device memory is released and allocated.
 void dispatcher_main(rts_server *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (!s->stop) {
@@ -250,20 +312,23 @@ void dispatcher_main(rts_server *s) {
                     }
                     Resident *res = find_resident(s, *r);
                     if (!res) {
-                        if (blocked || s->in_use - idle_bytes(s) + r->footprint > s->budget) {
+                        // one pipeline is prepared at a time; its memory is
+                        // reserved now and the request returns to the head
+                        // of the queue once the allocator thread has built it
+                        if (blocked || s->preparing > 0 || s->in_use - idle_bytes(s) + r->footprint > s->budget) {
                             blocked = true;
                             ++it;
                             continue;
                         }
-                        evict_for(s, r->footprint);
-                        if (int e = make_resident(s, r->req.pipe, r->req.stream_id, &res)) {
-                            r->st.status = e;
-                            r->done = true;
-                            it = s->queue.erase(it);
-                            s->cv_done.notify_all();
-                            continue;
-                        }
-                        s->in_use += res->footprint;
+                        AllocJob job;
+                        take_victims(s, r->footprint, job.victims);
+                        s->in_use += r->footprint;
+                        job.r = r;
+                        s->jobs.push_back(std::move(job));
+                        s->preparing++;
+                        it = s->queue.erase(it);
+                        s->cv_alloc.notify_one();
+                        continue;
                     }
                     res->busy = true;
                     r->started = true;
@@ -296,6 +361,7 @@ rts_status rts_server_create(size_t budget_bytes, int slots, rts_server **out) {
     s->assigned_res.assign(slots, nullptr);
     for (int i = 0; i < slots; i++) s->workers.emplace_back(worker_main, s, i);
     s->dispatcher = std::thread(dispatcher_main, s);
+    s->allocator = std::thread(allocator_main, s);
     *out = s;
     return RTS_OK;
 }
@@ -313,7 +379,9 @@ rts_status rts_server_destroy(rts_server *s) {
     }
     s->cv_work.notify_all();
     s->cv_done.notify_all();
+    s->cv_alloc.notify_all();
     s->dispatcher.join();
+    s->allocator.join();
     for (auto &t : s->workers) t.join();
     for (auto &res : s->residents) {
         rts_pipeline_destroy(res->pipe);

@@ -10,9 +10,35 @@ from __future__ import annotations
 
 import ctypes as C
 import dataclasses
+import os
+import sys
 import time
 
 import numpy as np
+
+
+def hw_queues_for(slots: int) -> int:
+    """Hardware queues a serving process wants: room for every slot stream
+    and the process's other streams (twice the slots), within the 32 the
+    runtime allows."""
+    return min(32, 2 * slots)
+
+
+if __name__ == "__main__":
+    # The serving program gives every slot's HIP stream a hardware queue of
+    # its own.  HIP spreads streams round-robin over GPU_MAX_HW_QUEUES queues
+    # (4 by default); a camera frame whose stream shares a queue with a
+    # burst's stream waits, in queue order, behind the burst's queued kernels
+    # (0.1 s and more at 40 bursts/s).  The runtime reads the variable when
+    # it starts, so it is set before the library loads.
+    _slots = 8
+    for _i, _a in enumerate(sys.argv):
+        if _a == "--slots" and _i + 1 < len(sys.argv):
+            _slots = int(sys.argv[_i + 1])
+        elif _a.startswith("--slots="):
+            _slots = int(_a.split("=", 1)[1])
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < hw_queues_for(_slots):
+        os.environ["GPU_MAX_HW_QUEUES"] = str(hw_queues_for(_slots))
 
 from . import _lib as L
 from ._lib import PipelineDesc, RtsError, call, lib
@@ -283,7 +309,8 @@ def main(argv=None) -> int:
                                                    "min": float(np.min(burst_agree))} if burst_agree else None
     c = srv.counters()
     out["server"] = {"budget_bytes": c.budget_bytes, "in_use_bytes": c.in_use_bytes, "resident": c.resident,
-                     "slots": args.slots, "oneoff_limit": min(args.oneoff_limit, args.slots)}
+                     "slots": args.slots, "oneoff_limit": min(args.oneoff_limit, args.slots),
+                     "gpu_max_hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))}
     out["input_generation_s"] = gen_s
     srv.close()
     print(json.dumps(out))
