@@ -264,6 +264,24 @@ def test_cross_short_arm_segments(W, H, D, seed):
     assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT | L.RTS_CROSS_ARMS_LE17)), A)
 
 
+@pytest.mark.parametrize("W,H,D,seed", [(64, 1, 8, 41), (66, 17, 6, 42), (72, 52, 8, 43), (33, 52, 8, 44),
+                                         (130, 87, 16, 45), (65, 140, 12, 46)])
+def test_cross_short_arm_edges(W, H, D, seed):
+    """Shapes at the short-arm sweep's edges: one row, fewer rows than one
+    step, a last step that completes more than 32 output rows (52, 87, 140:
+    the rows past the step's 32 are computed after the walk), and widths
+    below, at and just above one 64-column tile."""
+    p = _pair(W, H, D, seed)
+    gl, gr = p.gray()
+    c = oracle.combine(oracle.hamming(oracle.census(gl, 4, 3), oracle.census(gr, 4, 3), 4, 3, D),
+                       oracle.ad_color(p.left, p.right, D, 1), 30.0, 10.0)
+    P = oracle.hprefix(c)
+    al, hl, vl = S.find_cross(p.left)
+    ar, hr, vr = S.find_cross(p.right)
+    A = oracle.cross_v(P, hl, hr, vl, vr)
+    assert np.array_equal(pm(S.cross_v(dm(P), al, ar, L.RTS_CROSS_EXACT | L.RTS_CROSS_ARMS_LE17)), A)
+
+
 def test_cross_short_arm_special_values():
     """Denormal, infinite and NaN support sums take the quotient's IEEE
     division path; zeros and ordinary values the reciprocal path.  Both must
