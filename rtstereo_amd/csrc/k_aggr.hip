@@ -835,9 +835,11 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
     // non-temporal stores: the aggregated volume streams out to the next
     // kernel (measured 1.03 -> 1.00 ms here and 0.30 -> 0.28 ms for the WTA
     // that reads it, at 1080p/192)
+    // (buffer stores: the row as a scalar offset; cache policy 2 = nt)
+    const auto rsO = __builtin_amdgcn_make_buffer_rsrc((void *)(out + base), 0, pbytes, BUF_DWORD3);
     auto store_row = [&](int y, float v) {
-        float *orow = out + base + (size_t)y * W;
-        __builtin_nontemporal_store(v, (float *)((char *)orow + (uint32_t)x * 4u));
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rsO, (uint32_t)x * 4u, (uint32_t)y * (uint32_t)W * 4u,
+                                              2);
     };
     float res[KR];                 // this wave's outputs of the last step:
     int res_y = 0, res_end = 0;    // rows res_y + NW*i below res_end
