@@ -258,6 +258,10 @@ struct rts_pipeline {
     };
     bool graphs = false;
     RunKey gseen, gkey;
+    // second stream for the right view's preparation (gray, packed colour,
+    // arms, census), which runs beside the left view's (fork / join events)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraphExec_t gexec = nullptr;
     float *gfinal = nullptr;
     // per-stage timing (HIP events recorded on the run's stream)
@@ -399,8 +403,13 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
                           ((d.cost1 == RTS_COST_CENSUS && (d.cost2 == 0 || d.cost2 == RTS_COST_AD)) ||
                            (d.cost1 == RTS_COST_AD && (d.cost2 == 0 || d.cost2 == RTS_COST_CENSUS)));
     p->fused_front = front_ok;
-    if (d.aggr == RTS_AGGR_CROSS)
+    if (d.aggr == RTS_AGGR_CROSS) {
         for (int v = 0; v < 2; v++) A((void **)&p->pcol[v], P * 4);
+        if (st == RTS_OK && (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
+                             hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                             hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess))
+            st = fail(RTS_ERR_HIP, "side stream creation failed");
+    }
     if (front_ok && st == RTS_OK) {
         float hC[65], hA[766];
         const bool comb = d.cost2 != 0;
@@ -446,6 +455,9 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
     (void)hipFree(p->tabA);
     for (int i = 0; i < 8; i++)
         if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->side) (void)hipStreamDestroy(p->side);
     delete p;
     return RTS_OK;
 }
@@ -526,7 +538,40 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
     const uint8_t *gray[2] = {left_gray, right_gray};
     const bool need_gray = uses(d, RTS_COST_CENSUS) || uses(d, RTS_COST_RANK) || uses(d, RTS_COST_LOG) ||
                            uses(d, RTS_COST_NCC);
-    for (int v = 0; v < 2; v++)
+    // Census + cross with the fused front end and timing off: each view's
+    // gray, packed colour, arms and census run as one chain, the right
+    // view's on the side stream, so that the two chains of small kernels
+    // overlap; the front end waits for both.
+    const bool census_used = uses(d, RTS_COST_CENSUS);
+    bool views_done = false;
+    if (!p->timing && p->side && p->fused_front && d.aggr == RTS_AGGR_CROSS && d.cross_L <= 17) {
+        RTS_HIP(hipEventRecord(p->ev_fork, s));
+        RTS_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+        int e = RTS_OK;
+        for (int v = 0; v < 2 && e == RTS_OK; v++) {
+            hipStream_t sv = v ? p->side : s;
+            if (need_gray && !gray[v]) {
+                e = rts_rgb_to_gray(rgb[v], p->gray[v], W, H, sv);
+                gray[v] = p->gray[v];
+            }
+            if (e == RTS_OK) {
+                k_pack_rgb<<<(unsigned)div_up((long)W * H, 256), 256, 0, sv>>>(rgb[v], p->pcol[v], (size_t)W * H);
+                const hipError_t le = hipGetLastError();
+                if (le != hipSuccess) e = fail(RTS_ERR_HIP, "launch: %s", hipGetErrorString(le));
+            }
+            if (e == RTS_OK) e = find_cross_packed(p->pcol[v], p->arms[v], W, H, d.cross_L, d.cross_tao, sv);
+            if (e == RTS_OK && census_used)
+                e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, sv);
+        }
+        // joined after a failure too, so that a capture never ends with the
+        // side stream forked
+        const hipError_t j1 = hipEventRecord(p->ev_join, p->side);
+        const hipError_t j2 = hipStreamWaitEvent(s, p->ev_join, 0);
+        if (e != RTS_OK) return e;
+        if (j1 != hipSuccess || j2 != hipSuccess) return fail(RTS_ERR_HIP, "side stream join failed");
+        views_done = true;
+    }
+    for (int v = 0; v < 2 && !views_done; v++)
         if (need_gray && !gray[v]) {
             if (int e = rts_rgb_to_gray(rgb[v], p->gray[v], W, H, s)) return e;
             gray[v] = p->gray[v];
@@ -540,7 +585,7 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
         right_lab = p->lab[1];
     }
     float *cur = p->vol[0], *other = p->vol[1];
-    if (d.aggr == RTS_AGGR_CROSS)
+    if (d.aggr == RTS_AGGR_CROSS && !views_done)
         for (int v = 0; v < 2; v++) {
             k_pack_rgb<<<(unsigned)div_up((long)W * H, 256), 256, 0, s>>>(rgb[v], p->pcol[v], (size_t)W * H);
             RTS_LAUNCHED();
@@ -552,8 +597,8 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
         }
     bool prefixed = false;
     if (p->fused_front) {
-        const bool census = uses(d, RTS_COST_CENSUS), ad = uses(d, RTS_COST_AD);
-        if (census)
+        const bool census = census_used, ad = uses(d, RTS_COST_AD);
+        if (census && !views_done)
             for (int v = 0; v < 2; v++)
                 if (int e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, s)) return e;
         // the packed colour images were made with the cross arms above
