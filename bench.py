@@ -50,8 +50,8 @@ STAGE_KERNEL = {"cost": "k_front_hscan", "aggr": "k_cross_sweep", "wta": "k_wta"
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
 # the copy kernel's reading of the same counters (FETCH x2.000, WRITE x1.000);
 # profiles/census_cross_1080p_d192.md holds the runs.
-STAGE_TRAFFIC = {"cost": (24419.3 * 2 + 1555217.0) * 1024, "aggr": (1231438.9 * 2 + 1555200.0) * 1024,
-                 "wta": (803715.5 * 2 + 4050.0) * 1024}
+STAGE_TRAFFIC = {"cost": (24425.3 * 2 + 1555209.2) * 1024, "aggr": (1209324.0 * 2 + 1555200.1) * 1024,
+                 "wta": (803974.9 * 2 + 4050.0) * 1024}
 
 
 def cpu_model() -> str:
