@@ -260,6 +260,9 @@ struct rts_pipeline {
     RunKey gseen, gkey;
     // second stream for the right view's preparation (gray, packed colour,
     // arms, census), which runs beside the left view's (fork / join events)
+    // in captured graphs; created when graph replay is enabled, so that
+    // eagerly launched pipelines (a server's, many per process) add no
+    // streams
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -403,13 +406,8 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
                           ((d.cost1 == RTS_COST_CENSUS && (d.cost2 == 0 || d.cost2 == RTS_COST_AD)) ||
                            (d.cost1 == RTS_COST_AD && (d.cost2 == 0 || d.cost2 == RTS_COST_CENSUS)));
     p->fused_front = front_ok;
-    if (d.aggr == RTS_AGGR_CROSS) {
+    if (d.aggr == RTS_AGGR_CROSS)
         for (int v = 0; v < 2; v++) A((void **)&p->pcol[v], P * 4);
-        if (st == RTS_OK && (hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking) != hipSuccess ||
-                             hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming) != hipSuccess ||
-                             hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming) != hipSuccess))
-            st = fail(RTS_ERR_HIP, "side stream creation failed");
-    }
     if (front_ok && st == RTS_OK) {
         float hC[65], hA[766];
         const bool comb = d.cost2 != 0;
@@ -526,7 +524,8 @@ static int launch_front(rts_pipeline *p, float *P, int W, int H, int D, int vec,
 // stream capture).
 static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
                                   const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
-                                  const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask, void *stream) {
+                                  const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask, void *stream,
+                                  bool fork_views = false) {
     const rts_pipeline_desc &d = p->d;
     const int W = d.width, H = d.height, D = d.disp;
     hipStream_t s = S(stream);
@@ -538,13 +537,13 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
     const uint8_t *gray[2] = {left_gray, right_gray};
     const bool need_gray = uses(d, RTS_COST_CENSUS) || uses(d, RTS_COST_RANK) || uses(d, RTS_COST_LOG) ||
                            uses(d, RTS_COST_NCC);
-    // Census + cross with the fused front end and timing off: each view's
-    // gray, packed colour, arms and census run as one chain, the right
-    // view's on the side stream, so that the two chains of small kernels
-    // overlap; the front end waits for both.
+    // Census + cross with the fused front end, while a graph is captured:
+    // each view's gray, packed colour, arms and census run as one chain, the
+    // right view's on the side stream, so that the two chains of small
+    // kernels overlap; the front end waits for both.
     const bool census_used = uses(d, RTS_COST_CENSUS);
     bool views_done = false;
-    if (!p->timing && p->side && p->fused_front && d.aggr == RTS_AGGR_CROSS && d.cross_L <= 17) {
+    if (fork_views && !p->timing && p->side && p->fused_front && d.aggr == RTS_AGGR_CROSS && d.cross_L <= 17) {
         RTS_HIP(hipEventRecord(p->ev_fork, s));
         RTS_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
         int e = RTS_OK;
@@ -694,7 +693,7 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
     }
     RTS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int e = pipeline_launch(p, left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,
-                                  stream);
+                                  stream, true);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(s, &g);
     if (e != RTS_OK || ce != hipSuccess) {
@@ -715,6 +714,11 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
 
 extern "C" rts_status rts_pipeline_enable_graphs(rts_pipeline *p, int on) {
     RTS_CHECK(p != nullptr, "null pipeline");
+    if (on && !p->side && p->d.aggr == RTS_AGGR_CROSS) {
+        RTS_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+        RTS_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+        RTS_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    }
     p->graphs = on != 0;
     if (!p->graphs && p->gexec) {
         RTS_HIP(hipGraphExecDestroy(p->gexec));
