@@ -459,8 +459,10 @@ __global__ void __launch_bounds__(256) k_cost_rank_log_ncc(const uint8_t *__rest
 // columns its lanes' windows touch as 16-bit pairs in LDS and forms the
 // row's horizontal window sums of a, a^2, b, b^2, ab, a^2b, ab^2, a^2b^2;
 // the last 2HR+1 of them stay in a register ring, so the vertical sums move
-// by one add and one subtract per row.  The sums are the exact integers of
-// k_cost_rank_log_ncc, so both kernels write identical values.
+// by one add and one subtract per row.  The window sums are the exact
+// integers of k_cost_rank_log_ncc; the quotient differs from that kernel's
+// in its last bits (double intermediates and a hardware reciprocal square
+// root here, an exact integer Z and a double square root there).
 constexpr int NS_SEG = 64;
 
 struct NccSums {
@@ -479,15 +481,19 @@ __device__ __forceinline__ void ncc_add(NccSums &s, const NccSums &h, const NccS
     s.saabb += h.saabb - o.saabb;
 }
 
-// ncc_exact with the quotient in float (the integers Z and N*nom are exact;
-// the float rounding of the quotient is ~1e-7 relative).
+// The NCC value from the window sums in double arithmetic (for a 7x7 window
+// every product in Y and nomN is below 2^53 and exact; N*Y and (Sa*Sb)^2 may
+// round, ~1e-16 relative) and a float quotient with the hardware reciprocal
+// square root (~1e-7 relative).  A window with a flat side has nomN == 0
+// exactly, so it gives 0 either way.
 __device__ __forceinline__ float ncc_fast(long N, long Sa, long Saa, long Sb, long Sbb, long Sab, long Saab, long Sabb,
                                           unsigned long Saabb) {
-    const long Y = N * N * (long)Saabb - 2 * N * (Sb * Saab + Sa * Sabb) + Sb * Sb * Saa + Sa * Sa * Sbb +
-                   4 * Sa * Sb * Sab;
-    const long Z = N * Y - 3 * (Sa * Sb) * (Sa * Sb);
-    const long nomN = N * Sab - Sa * Sb;
-    return Z > 0 ? (float)N * (float)nomN * __builtin_amdgcn_rsqf((float)N * (float)Z) : 0.0f;
+    const double n = (double)N, a = (double)Sa, aa = (double)Saa, b = (double)Sb, bb = (double)Sbb;
+    const double ab = (double)Sab, aab = (double)Saab, abb = (double)Sabb, aabb = (double)Saabb;
+    const double Y = n * n * aabb - 2.0 * n * (b * aab + a * abb) + b * b * aa + a * a * bb + 4.0 * a * b * ab;
+    const double Z = n * Y - 3.0 * (a * b) * (a * b);
+    const double nomN = n * ab - a * b;
+    return Z > 0 ? (float)n * (float)nomN * __builtin_amdgcn_rsqf((float)n * (float)Z) : 0.0f;
 }
 
 template <int HR, int WR, int DCH>
