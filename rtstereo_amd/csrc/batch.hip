@@ -287,8 +287,11 @@ void carve(rts_batch *b, Chunk &c, const rts_batch_item *items) {
 }
 
 // Times the candidates of each tuned parameter on the largest pair of the
-// first chunk and keeps the fastest: the box strip height (16, 32, 64 rows)
-// and the cost kernel's disparities per block (8, 16, 32).
+// first chunk (the best of three runs after a warm-up): the box strip
+// height (16, 32, 64 rows) and the cost kernel's disparities per block (8,
+// 16, 32).  A larger candidate replaces a smaller one only when it is 2%
+// faster: on the largest pair they often tie, and the smaller ones give the
+// batch's smaller pairs more blocks.
 int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s) {
     size_t big = 0;
     for (size_t i = 1; i < c.items.size(); i++)
@@ -301,17 +304,21 @@ int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s)
     RTS_HIP(hipEventCreate(&e1));
     int st = prep_pair(b, items[one.items[0]], one.bufs[0], s);
     const int dchs[3] = {8, 16, 32}, tys[3] = {16, 32, 64};
+    constexpr int REPS = 4;            // the first run warms up
+    constexpr float SWITCH = 0.98f;    // a larger candidate must be 2% faster
     float best = 1e30f;
     for (int k = 0; k < 3 && st == RTS_OK; k++) {
-        float ms = 0.0f;
-        for (int rep = 0; rep < 2 && st == RTS_OK; rep++) {   // first run warms up
+        float ms = 1e30f;
+        for (int rep = 0; rep < REPS && st == RTS_OK; rep++) {
+            float t = 0.0f;
             st = hipEventRecord(e0, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
             if (st == RTS_OK) st = cost_pair(b, items[one.items[0]], one.bufs[0], dchs[k], s);
             if (st == RTS_OK) st = hipEventRecord(e1, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
-            if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+            if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+            if (rep > 0 && t < ms) ms = t;
         }
         b->tune_ms[k] = ms;
-        if (ms < best) {
+        if (k == 0 || ms < best * SWITCH) {
             best = ms;
             b->cost_dch = dchs[k];
         }
@@ -319,15 +326,17 @@ int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s)
     best = 1e30f;
     for (int k = 0; k < 3 && st == RTS_OK; k++) {
         st = build_box_table(one, items, tys[k], s);
-        float ms = 0.0f;
-        for (int rep = 0; rep < 2 && st == RTS_OK; rep++) {
+        float ms = 1e30f;
+        for (int rep = 0; rep < REPS && st == RTS_OK; rep++) {
+            float t = 0.0f;
             st = hipEventRecord(e0, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
             if (st == RTS_OK) st = launch_box(one, tys[k], b->base.box_r, s);
             if (st == RTS_OK) st = hipEventRecord(e1, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
-            if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+            if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+            if (rep > 0 && t < ms) ms = t;
         }
         b->tune_ms[3 + k] = ms;
-        if (ms < best) {
+        if (k == 0 || ms < best * SWITCH) {
             best = ms;
             b->box_ty = tys[k];
         }
