@@ -41,16 +41,16 @@ struct BoxPair {
 // Work item t of pair p: d = t / (ny*nx), strip = (t / nx) % ny, tile = t % nx.
 constexpr int BQ_MAXP = 256;   // pairs per chunk
 
-template <int TY>
+template <int TY, int R>
 __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ pairs, int npairs, unsigned ntiles,
-                                                   unsigned *__restrict__ counter, int r) {
-    __shared__ float tile[TY + 2 * BQ_RMAX][64 + 2 * BQ_RMAX + 1];
-    __shared__ float hs[TY + 2 * BQ_RMAX][64];
+                                                   unsigned *__restrict__ counter) {
+    constexpr int E = 2 * R + 1, TH = TY + 2 * R, TW = 64 + 2 * R;
+    __shared__ float tile[TH][TW + 1];
+    __shared__ float hs[TH][65];
     __shared__ unsigned first[BQ_MAXP];   // each pair's first work item
     __shared__ unsigned item;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int e = 2 * r + 1;
-    const float area = (float)(e * e);
+    const float area = (float)(E * E);
     for (int i = threadIdx.x; i < npairs; i += 256) first[i] = pairs[i].tile0;
     // The next work item is requested while the current one is processed.
     unsigned nxt = 0;
@@ -72,19 +72,18 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
         const int xb = (int)(u % bp.nx), sb = (int)((u / bp.nx) % bp.ny), d = (int)(u / ((unsigned)bp.nx * bp.ny));
         const int W = bp.W, H = bp.H, x0 = xb * 64, y0 = sb * TY;
         const size_t base = (size_t)d * plane(W, H);
-        const int th = TY + 2 * r, tw = 64 + 2 * r;
-        const bool inner = x0 - r >= 0 && x0 + 64 + r <= W && y0 - r >= 0 && y0 + TY + r <= H;
+        const bool inner = x0 - R >= 0 && x0 + 64 + R <= W && y0 - R >= 0 && y0 + TY + R <= H;
         const float *src = bp.in + base;
         // All of a thread's loads are issued before its LDS stores.
-        constexpr int NB = ((TY + 2 * BQ_RMAX) * (64 + 2 * BQ_RMAX) + 255) / 256;
+        constexpr int NB = (TH * TW + 255) / 256;
         float v[NB];
 #pragma unroll
         for (int k = 0; k < NB; k++) {
             const int i = threadIdx.x + 256 * k;
-            const int rr = i / tw, cc = i - rr * tw;
+            const int rr = i / TW, cc = i - rr * TW;
             float x = 0.0f;
-            if (rr < th) {
-                const int yy = y0 - r + rr, xx = x0 - r + cc;
+            if (rr < TH) {
+                const int yy = y0 - R + rr, xx = x0 - R + cc;
                 x = inner ? src[(size_t)yy * W + xx] : src[(size_t)wrapn(yy, H) * W + wrapn(xx, W)];
             }
             v[k] = x;
@@ -92,27 +91,50 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
 #pragma unroll
         for (int k = 0; k < NB; k++) {
             const int i = threadIdx.x + 256 * k;
-            const int rr = i / tw, cc = i - rr * tw;
-            if (rr < th) tile[rr][cc] = v[k];
+            const int rr = i / TW, cc = i - rr * TW;
+            if (rr < TH) tile[rr][cc] = v[k];
         }
         __syncthreads();
-        for (int rr = ty; rr < th; rr += 4) {
-            float h = 0.0f;
-            for (int dx = 0; dx < e; dx++) h += tile[rr][tx + dx];
-            hs[rr][tx] = h;
-        }
-        __syncthreads();
-        const int x = x0 + tx;
-        if (x < W)
-            for (int ly = ty; ly < TY; ly += 4) {
-                const int y = y0 + ly;
-                if (y >= H) break;
-                float acc = 0.0f;
-                for (int dy = 0; dy < e; dy++) acc += hs[ly + dy][tx];
-                stream_store(&bp.out[base + (size_t)y * W + x], __fdiv_rn(acc, area));
+        // Horizontal sums: a thread takes a 16-column piece of one row, keeps
+        // its 16 + 2R inputs in registers and adds each output's E taps in
+        // column order (one LDS read per input instead of E per output).
+        for (int task = threadIdx.x; task < TH * 4; task += 256) {
+            const int rr = task >> 2, c0 = (task & 3) * 16;
+            float w[16 + 2 * R];
+#pragma unroll
+            for (int i = 0; i < 16 + 2 * R; i++) w[i] = tile[rr][c0 + i];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                float h = 0.0f;
+#pragma unroll
+                for (int dx = 0; dx < E; dx++) h += w[j + dx];
+                hs[rr][c0 + j] = h;
             }
+        }
+        __syncthreads();
+        // Vertical sums: a thread takes TY/4 consecutive output rows of one
+        // column, its TY/4 + 2R horizontal sums in registers, taps in row
+        // order.
+        constexpr int RPT = TY / 4;
+        const int x = x0 + tx;
+        if (x < W) {
+            float cv[RPT + 2 * R];
+#pragma unroll
+            for (int i = 0; i < RPT + 2 * R; i++) cv[i] = hs[ty * RPT + i][tx];
+#pragma unroll
+            for (int j = 0; j < RPT; j++) {
+                const int y = y0 + ty * RPT + j;
+                if (y < H) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int dy = 0; dy < E; dy++) acc += cv[j + dy];
+                    stream_store(&bp.out[base + (size_t)y * W + x], __fdiv_rn(acc, area));
+                }
+            }
+        }
     }
 }
+
 
 __global__ void k_zero_u32(unsigned *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -179,14 +201,29 @@ void free_chunk(Chunk *c) {
     c->counter = nullptr;
 }
 
+template <int R>
+void launch_box_r(const Chunk &c, int ty, int nb, hipStream_t s) {
+    switch (ty) {
+        case 16: k_box_queue<16, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter); break;
+        case 64: k_box_queue<64, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter); break;
+        default: k_box_queue<32, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter); break;
+    }
+}
+
 int launch_box(const Chunk &c, int ty, int r, hipStream_t s) {
     const int nb = std::max(1, std::min<int>((int)std::min<unsigned>(c.ntiles, 1u << 30), 256 * 8));
     k_zero_u32<<<1, 64, 0, s>>>(c.counter, 1);
     RTS_LAUNCHED();
-    switch (ty) {
-        case 16: k_box_queue<16><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter, r); break;
-        case 64: k_box_queue<64><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter, r); break;
-        default: k_box_queue<32><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter, r); break;
+    switch (r) {
+        case 0: launch_box_r<0>(c, ty, nb, s); break;
+        case 1: launch_box_r<1>(c, ty, nb, s); break;
+        case 2: launch_box_r<2>(c, ty, nb, s); break;
+        case 3: launch_box_r<3>(c, ty, nb, s); break;
+        case 4: launch_box_r<4>(c, ty, nb, s); break;
+        case 5: launch_box_r<5>(c, ty, nb, s); break;
+        case 6: launch_box_r<6>(c, ty, nb, s); break;
+        case 7: launch_box_r<7>(c, ty, nb, s); break;
+        default: launch_box_r<BQ_RMAX>(c, ty, nb, s); break;
     }
     RTS_LAUNCHED();
     return RTS_OK;
