@@ -13,9 +13,75 @@ namespace rts {
 
 // ---------------------------------------------------------------- box
 
-// seq/ConstBlockAggregation.cpp:11-34, out of place.  One block = one d
-// plane tile of 64 x BOX_TY outputs; the (64+2r) x (BOX_TY+2r) input tile is
-// staged in LDS and every output sums its window in row-major order.
+// seq/ConstBlockAggregation.cpp:11-34, out of place.  Every output sums its
+// (2r+1)^2 window in the reference's row-major order (rows outer, columns
+// inner, one float accumulator) and divides by the window area.
+
+// Radii 0..BOX_RT: one block = one d plane tile of 64 x BOXR_TY outputs.  The
+// (64+2R) x (BOXR_TY+2R) input tile is staged in LDS; a thread owns one
+// column and BOXR_RPT consecutive rows, reads each input row's 2R+1 taps
+// once and adds every tap to the outputs whose window holds that row, so a
+// tap costs one LDS read per BOXR_RPT outputs instead of one per output.
+// Each accumulator still receives its taps rows-ascending, columns-ascending.
+constexpr int BOXR_TY = 32;
+constexpr int BOXR_RPT = BOXR_TY / 4;
+constexpr int BOX_RT = 8;
+
+template <int R>
+__global__ void __launch_bounds__(256) k_box_rw(const float *__restrict__ in, float *__restrict__ out, int W, int H) {
+    constexpr int E = 2 * R + 1, TH = BOXR_TY + 2 * R, TW = 64 + 2 * R;
+    __shared__ float tile[TH][TW];
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * BOXR_TY, d = blockIdx.z;
+    const size_t base = (size_t)d * plane(W, H);
+    const float *src = in + base;
+    const bool inner = x0 - R >= 0 && x0 + 64 + R <= W && y0 - R >= 0 && y0 + BOXR_TY + R <= H;
+    // All of a thread's loads are issued before its LDS stores.
+    constexpr int NB = (TH * TW + 255) / 256;
+    float v[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        const int i = threadIdx.x + 256 * k;
+        const int rr = i / TW, cc = i - rr * TW;
+        float x = 0.0f;
+        if (rr < TH) {
+            const int yy = y0 - R + rr, xx = x0 - R + cc;
+            x = inner ? src[(size_t)yy * W + xx] : src[(size_t)wrapn(yy, H) * W + wrapn(xx, W)];
+        }
+        v[k] = x;
+    }
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        const int i = threadIdx.x + 256 * k;
+        const int rr = i / TW, cc = i - rr * TW;
+        if (rr < TH) tile[rr][cc] = v[k];
+    }
+    __syncthreads();
+    const int tx = threadIdx.x & 63, r0 = (threadIdx.x >> 6) * BOXR_RPT;
+    float acc[BOXR_RPT];
+#pragma unroll
+    for (int i = 0; i < BOXR_RPT; i++) acc[i] = 0.0f;
+#pragma unroll
+    for (int rr = 0; rr < BOXR_RPT + 2 * R; rr++) {
+#pragma unroll
+        for (int dx = 0; dx < E; dx++) {
+            const float t = tile[r0 + rr][tx + dx];
+#pragma unroll
+            for (int i = 0; i < BOXR_RPT; i++)
+                if (rr >= i && rr < i + E) acc[i] += t;
+        }
+    }
+    const int x = x0 + tx;
+    if (x >= W) return;
+    const float area = (float)(E * E);
+#pragma unroll
+    for (int i = 0; i < BOXR_RPT; i++) {
+        const int y = y0 + r0 + i;
+        if (y < H) stream_store(&out[base + (size_t)y * W + x], __fdiv_rn(acc[i], area));
+    }
+}
+
+// Radii BOX_RT+1..BOX_RMAX: one block = one d plane tile of 64 x BOX_TY
+// outputs, every tap read from the LDS tile.
 constexpr int BOX_TY = 16;
 constexpr int BOX_RMAX = 16;
 
@@ -952,7 +1018,21 @@ rts_status rts_box(const float *in, float *out, int W, int H, int D, int r, void
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(r >= 0 && r < W && r < H, "box radius %d does not fit the image", r);
     RTS_CHECK(in != out, "rts_box is out of place");
-    if (r <= BOX_RMAX) {
+    if (r <= BOX_RT) {
+        const dim3 grid(div_up(W, 64), div_up(H, BOXR_TY), D);
+        hipStream_t s = S(stream);
+        switch (r) {
+            case 0: k_box_rw<0><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            case 1: k_box_rw<1><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            case 2: k_box_rw<2><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            case 3: k_box_rw<3><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            case 4: k_box_rw<4><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            case 5: k_box_rw<5><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            case 6: k_box_rw<6><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            case 7: k_box_rw<7><<<grid, 256, 0, s>>>(in, out, W, H); break;
+            default: k_box_rw<BOX_RT><<<grid, 256, 0, s>>>(in, out, W, H); break;
+        }
+    } else if (r <= BOX_RMAX) {
         dim3 grid(div_up(W, 64), div_up(H, BOX_TY), D), block(64, 4);
         k_box_f32<<<grid, block, 0, S(stream)>>>(in, out, W, H, r);
     } else {

@@ -107,7 +107,7 @@ def test_combine():
 
 
 @pytest.mark.parametrize("W,H,D,seed", SIZES)
-@pytest.mark.parametrize("r", [1, 4, 5, 17])
+@pytest.mark.parametrize("r", [0, 1, 4, 5, 8, 12, 17])
 def test_box(W, H, D, seed, r):
     if r >= min(W, H):
         pytest.skip("radius larger than the image")
@@ -116,6 +116,14 @@ def test_box(W, H, D, seed, r):
     assert np.array_equal(pm(S.box(dm(c), r)), oracle.box(c, r))
     f = oracle.ad_float(oracle.convolve(p.gray()[0], oracle.LOG5, 2, 2), oracle.convolve(p.gray()[1], oracle.LOG5, 2, 2), D)
     assert np.array_equal(pm(S.box(dm(f), r)), oracle.box(f, r))
+
+
+@pytest.mark.parametrize("r", [2, 3, 6, 7])
+def test_box_interior_tiles(r):
+    """Tiles whose window lies inside the image (no wrap) next to border tiles."""
+    p = _pair(257, 131, 5, 12)
+    c = oracle.ad_color(p.left, p.right, 5, 1, 20.0, False)
+    assert np.array_equal(pm(S.box(dm(c), r)), oracle.box(c, r))
 
 
 @pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
