@@ -157,6 +157,40 @@ def aw_timing(stream) -> dict:
     return res
 
 
+def config_timing(stream, frames: int = 50) -> dict:
+    """The two other single-pair workloads: 384x288 plain (AD trunc 20, 9x9
+    box, D=16) and 1280x720 census + cross-based aggregation with four H/V
+    passes (D=128).  Mean frame time over `frames` graph-replayed frames
+    after two warm-up frames, and agreement with the generator's ground
+    truth."""
+    from rtstereo_amd import gen, pipeline as PL
+    from rtstereo_amd.device import DeviceArray, Event
+    res = {}
+    for name, pair, preset, (Wc, Hc, Dc), over in (
+            ("plain_384x288_d16", gen.plain_384x288(), "plain", (384, 288, 16), {}),
+            ("census_cross_720p_d128_4pass", gen.census_cross_720p(), "census_cross", (1280, 720, 128),
+             {"cross_iters": 4})):
+        a, b = DeviceArray.from_host(pair.left), DeviceArray.from_host(pair.right)
+        pipe = PL.Pipeline(PL.preset(preset, Wc, Hc, Dc, **over))
+        pipe.enable_graphs(True)
+        for _ in range(2):
+            pipe.run(a, b, stream=stream.handle)
+        stream.sync()
+        agree = float((pipe.dl.download(stream.handle) == pair.gt).mean())
+        e0, e1 = Event(), Event()
+        e0.record(stream.handle)
+        for _ in range(frames):
+            pipe.run(a, b, stream=stream.handle)
+        e1.record(stream.handle)
+        stream.sync()
+        ms = e0.elapsed_ms(e1) / frames
+        res[name] = {"frame_ms": ms, "pd_per_s": Wc * Hc * Dc / (ms / 1e3), "gt_agreement": agree}
+        pipe.close()
+        a.free()
+        b.free()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +199,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--no-aw", action="store_true", help="skip the adaptive-weight FP32/FP16 timing")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 384x288 plain and 720p 4-pass timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has one GPU, "
@@ -278,6 +313,8 @@ def main() -> int:
     }
     if not args.no_aw:
         out["aw"] = aw_timing(stream)
+    if not args.no_configs:
+        out["other_configs"] = config_timing(stream)
     if args.cpu_seconds > 0:
         cb, cbn = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb
