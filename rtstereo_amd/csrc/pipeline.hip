@@ -214,6 +214,31 @@ __global__ void k_pack_rgb(const uint8_t *__restrict__ rgb, uint32_t *__restrict
     if (i < n) out[i] = pack_rgb(rgb + 3 * i);
 }
 
+// Packed colour and gray level in one pass over the B,G,R image.
+__global__ void k_gray_pack(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ gray, uint32_t *__restrict__ pcol,
+                            size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t *q = rgb + 3 * i;
+    pcol[i] = pack_rgb(q);
+    gray[i] = gray_bgr(q[0], q[1], q[2]);
+}
+
+// Joins the side stream's arm maps into the run's stream: explicitly before
+// the vertical sweep reads them, or on the way out of a failed launch, so a
+// capture never ends with the side stream forked.
+struct ArmsJoin {
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    hipError_t join() {
+        if (!s) return hipSuccess;
+        const hipError_t e = hipStreamWaitEvent(s, ev, 0);
+        s = nullptr;
+        return e;
+    }
+    ~ArmsJoin() { (void)join(); }
+};
+
 // The AD value the colour AD kernel produces for integer channel sum s.
 static float ad_value(int s, int use_trunc, float trunc, int norm) {
     const float third = (float)1 / (float)3;
@@ -258,13 +283,15 @@ struct rts_pipeline {
     };
     bool graphs = false;
     RunKey gseen, gkey;
-    // second stream for the right view's preparation (gray, packed colour,
-    // arms, census), which runs beside the left view's (fork / join events)
-    // in captured graphs; created when graph replay is enabled, so that
+    // second stream for captured graphs: the right view's gray, packed
+    // colour and census run on it beside the left view's (fork / join
+    // events), then both views' arms beside the front end (ev_left: the
+    // left view's packed colour is ready; ev_arms: joined before the
+    // vertical sweep).  Created when graph replay is enabled, so that
     // eagerly launched pipelines (a server's, many per process) add no
-    // streams
+    // streams.
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_left = nullptr, ev_arms = nullptr;
     hipGraphExec_t gexec = nullptr;
     float *gfinal = nullptr;
     // per-stage timing (HIP events recorded on the run's stream)
@@ -455,6 +482,8 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
         if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
     if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
     if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+    if (p->ev_left) (void)hipEventDestroy(p->ev_left);
+    if (p->ev_arms) (void)hipEventDestroy(p->ev_arms);
     if (p->side) (void)hipStreamDestroy(p->side);
     delete p;
     return RTS_OK;
@@ -538,27 +567,32 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
     const bool need_gray = uses(d, RTS_COST_CENSUS) || uses(d, RTS_COST_RANK) || uses(d, RTS_COST_LOG) ||
                            uses(d, RTS_COST_NCC);
     // Census + cross with the fused front end, while a graph is captured:
-    // each view's gray, packed colour, arms and census run as one chain, the
+    // each view's gray + packed colour and census run as one chain, the
     // right view's on the side stream, so that the two chains of small
-    // kernels overlap; the front end waits for both.
+    // kernels overlap; the front end waits for both.  The arms are read only
+    // by the vertical sweep: both views' arms then run on the side stream
+    // beside the front end and are joined before the sweep.
     const bool census_used = uses(d, RTS_COST_CENSUS);
     bool views_done = false;
-    if (fork_views && !p->timing && p->side && p->fused_front && d.aggr == RTS_AGGR_CROSS && d.cross_L <= 17) {
+    ArmsJoin arms_join;
+    if (fork_views && !p->timing && p->side && p->ev_fork && p->ev_join && p->ev_left && p->ev_arms &&
+        p->fused_front && d.aggr == RTS_AGGR_CROSS && d.cross_L <= 17) {
         RTS_HIP(hipEventRecord(p->ev_fork, s));
         RTS_HIP(hipStreamWaitEvent(p->side, p->ev_fork, 0));
         int e = RTS_OK;
+        const unsigned nb = (unsigned)div_up((long)W * H, 256);
         for (int v = 0; v < 2 && e == RTS_OK; v++) {
             hipStream_t sv = v ? p->side : s;
             if (need_gray && !gray[v]) {
-                e = rts_rgb_to_gray(rgb[v], p->gray[v], W, H, sv);
+                k_gray_pack<<<nb, 256, 0, sv>>>(rgb[v], p->gray[v], p->pcol[v], (size_t)W * H);
                 gray[v] = p->gray[v];
+            } else {
+                k_pack_rgb<<<nb, 256, 0, sv>>>(rgb[v], p->pcol[v], (size_t)W * H);
             }
-            if (e == RTS_OK) {
-                k_pack_rgb<<<(unsigned)div_up((long)W * H, 256), 256, 0, sv>>>(rgb[v], p->pcol[v], (size_t)W * H);
-                const hipError_t le = hipGetLastError();
-                if (le != hipSuccess) e = fail(RTS_ERR_HIP, "launch: %s", hipGetErrorString(le));
-            }
-            if (e == RTS_OK) e = find_cross_packed(p->pcol[v], p->arms[v], W, H, d.cross_L, d.cross_tao, sv);
+            const hipError_t le = hipGetLastError();
+            if (le != hipSuccess) e = fail(RTS_ERR_HIP, "launch: %s", hipGetErrorString(le));
+            if (e == RTS_OK && v == 0 && hipEventRecord(p->ev_left, s) != hipSuccess)
+                e = fail(RTS_ERR_HIP, "event record failed");
             if (e == RTS_OK && census_used)
                 e = rts_census(gray[v], p->census[v], W, H, d.census_hr, d.census_wr, sv);
         }
@@ -568,6 +602,16 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
         const hipError_t j2 = hipStreamWaitEvent(s, p->ev_join, 0);
         if (e != RTS_OK) return e;
         if (j1 != hipSuccess || j2 != hipSuccess) return fail(RTS_ERR_HIP, "side stream join failed");
+        RTS_HIP(hipStreamWaitEvent(p->side, p->ev_left, 0));
+        e = find_cross_packed(p->pcol[1], p->arms[1], W, H, d.cross_L, d.cross_tao, p->side);
+        if (e == RTS_OK) e = find_cross_packed(p->pcol[0], p->arms[0], W, H, d.cross_L, d.cross_tao, p->side);
+        const hipError_t j3 = hipEventRecord(p->ev_arms, p->side);
+        if (j3 == hipSuccess) {
+            arms_join.s = s;
+            arms_join.ev = p->ev_arms;
+        }
+        if (e != RTS_OK) return e;
+        if (j3 != hipSuccess) return fail(RTS_ERR_HIP, "side stream join failed");
         views_done = true;
     }
     for (int v = 0; v < 2 && !views_done; v++)
@@ -641,6 +685,7 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
             std::swap(cur, other);
             break;
         case RTS_AGGR_CROSS:
+            if (arms_join.join() != hipSuccess) return fail(RTS_ERR_HIP, "side stream join failed");
             for (int it = 0; it < d.cross_iters; it++) {
                 // the reference's dataflow: serial row prefix (fused into the
                 // front end for the first pass), then the vertical sweep
@@ -714,10 +759,10 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
 
 extern "C" rts_status rts_pipeline_enable_graphs(rts_pipeline *p, int on) {
     RTS_CHECK(p != nullptr, "null pipeline");
-    if (on && !p->side && p->d.aggr == RTS_AGGR_CROSS) {
-        RTS_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-        RTS_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
-        RTS_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    if (on && p->d.aggr == RTS_AGGR_CROSS) {
+        if (!p->side) RTS_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+        for (hipEvent_t *ev : {&p->ev_fork, &p->ev_join, &p->ev_left, &p->ev_arms})
+            if (!*ev) RTS_HIP(hipEventCreateWithFlags(ev, hipEventDisableTiming));
     }
     p->graphs = on != 0;
     if (!p->graphs && p->gexec) {

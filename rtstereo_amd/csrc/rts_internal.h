@@ -102,6 +102,12 @@ __device__ __forceinline__ int max_chan_delta(uint32_t a, uint32_t b) {
     return m;
 }
 
+// Gray level of a B,G,R pixel: fixed-point weights 29/150/77 over 256,
+// rounded (OpenCV's BGR2GRAY integer form).
+__device__ __forceinline__ uint8_t gray_bgr(uint32_t b, uint32_t g, uint32_t r) {
+    return (uint8_t)((29u * b + 150u * g + 77u * r + 128u) >> 8);
+}
+
 // Packed arm layout: byte 0 left, 1 right, 2 top, 3 bottom.
 __device__ __forceinline__ int arm_l(uint32_t a) { return (int)(a & 255u); }
 __device__ __forceinline__ int arm_r(uint32_t a) { return (int)((a >> 8) & 255u); }

@@ -64,8 +64,7 @@ __global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src, u
 __global__ void k_rgb_to_gray(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ gray, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t b = rgb[3 * i], g = rgb[3 * i + 1], r = rgb[3 * i + 2];
-    gray[i] = (uint8_t)((29u * b + 150u * g + 77u * r + 128u) >> 8);
+    gray[i] = gray_bgr(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
 }
 
 // 8-bit CIELab (D65) from interleaved B,G,R bytes, in FP64: L*255/100,

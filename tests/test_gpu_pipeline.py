@@ -140,6 +140,7 @@ def test_census_cross_1080p_properties(exact):
     ("census_cross", 96, 64, 24, {}),
     ("census_cross", 150, 70, 40, {"cross_iters": 2}),
     ("census_cross", 96, 64, 24, {"wta_literal": 1}),
+    ("census_cross", 80, 48, 16, {"cost1": L.RTS_COST_AD, "cost2": 0, "ad_trunc_enable": 1, "ad_trunc": 24.0}),
     ("plain", 96, 64, 16, {}),
     ("aw", 80, 48, 16, {"aw_r": 5}),
     ("rank_log_ncc", 96, 64, 24, {}),
