@@ -50,8 +50,8 @@ STAGE_KERNEL = {"cost": "k_front_hscan", "aggr": "k_cross_sweep", "wta": "k_wta"
 # HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
 # the copy kernel's reading of the same counters (FETCH x2.000, WRITE x1.000);
 # profiles/census_cross_1080p_d192.md holds the runs.
-STAGE_TRAFFIC = {"cost": (24425.3 * 2 + 1555209.2) * 1024, "aggr": (1209324.0 * 2 + 1555200.1) * 1024,
-                 "wta": (803974.9 * 2 + 4050.0) * 1024}
+STAGE_TRAFFIC = {"cost": (24424.9 * 2 + 1555212.8) * 1024, "aggr": (1215821.1 * 2 + 1555200.1) * 1024,
+                 "wta": (805732.9 * 2 + 4050.0) * 1024}
 
 
 def cpu_model() -> str:
@@ -194,8 +194,8 @@ def config_timing(stream, frames: int = 50) -> dict:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--no-aw", action="store_true", help="skip the adaptive-weight FP32/FP16 timing")
