@@ -156,6 +156,21 @@ rts_status rts_cost_rank_log_ncc_dch(const uint8_t *gray_ref, const uint8_t *gra
                                      const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol,
                                      int W, int H, int D, int hr, int wr, float l1, float l2, float l3, int dch,
                                      void *stream);
+/* Bytes of the workspace of rts_cost_rank_log_ncc_ws at W x H: a 16-byte
+ * record per pixel and view. */
+size_t rts_cost_rank_log_ncc_ws_bytes(int W, int H);
+/* rts_cost_rank_log_ncc_dch with a caller-owned workspace (16-byte aligned,
+ * at least rts_cost_rank_log_ncc_ws_bytes(W, H)).  For the 7x7 window
+ * (hr = wr = 3) a pre-pass stores each view's per-pixel gray window sums,
+ * rank and LoG value there, so the cost kernel forms only the window sums
+ * with products of both views per disparity; the output is identical to
+ * rts_cost_rank_log_ncc_dch's.  Other windows, or ws == NULL, run
+ * rts_cost_rank_log_ncc_dch.  The workspace is in use until the stream
+ * reaches the end of the call's work. */
+rts_status rts_cost_rank_log_ncc_ws(const uint8_t *gray_ref, const uint8_t *gray_tar, const uint8_t *rank_ref,
+                                    const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol,
+                                    int W, int H, int D, int hr, int wr, float l1, float l2, float l3, int dch,
+                                    void *ws, size_t ws_bytes, void *stream);
 
 /* ------------------------------------------------------------ aggregation */
 

@@ -92,6 +92,8 @@ _SIGS = {
     "rts_aw_workspace_bytes": [i32, i32, i32, i32],
     "rts_cost_rank_log_ncc": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, vp],
     "rts_cost_rank_log_ncc_dch": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, i32, vp],
+    "rts_cost_rank_log_ncc_ws": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, i32, vp, sz, vp],
+    "rts_cost_rank_log_ncc_ws_bytes": [i32, i32],
     "rts_pipeline_create": [C.POINTER(PipelineDesc), C.POINTER(vp)],
     "rts_pipeline_destroy": [vp],
     "rts_pipeline_run": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
@@ -105,7 +107,7 @@ _SIGS = {
 }
 _RESTYPES = {"rts_last_error": cp, "rts_version": cp, "rts_pipeline_defaults": None,
              "rts_pipeline_bytes": sz, "rts_pipeline_volume": vp,
-             "rts_aw_workspace_bytes": sz}
+             "rts_aw_workspace_bytes": sz, "rts_cost_rank_log_ncc_ws_bytes": sz}
 
 for _name, _args in _SIGS.items():
     if not hasattr(lib, _name):

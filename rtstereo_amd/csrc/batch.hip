@@ -144,6 +144,7 @@ __global__ void k_zero_u32(unsigned *p, int n) {
 struct PairBufs {
     uint8_t *gray[2], *rank[2];
     float *logimg[2], *vol, *agg;
+    void *ncc_ws;   // the cost kernel's per-pixel records of both views
 };
 
 struct Chunk {
@@ -162,7 +163,7 @@ size_t pair_bytes(const rts_batch_item &it) {
     const size_t P = (size_t)it.width * it.height;
     const size_t V = P * it.disp * sizeof(float);
     auto al = [](size_t n) { return (n + 255) & ~(size_t)255; };
-    return 2 * al(V) + 2 * (al(P) + al(P) + al(P * 4));
+    return 2 * al(V) + 2 * (al(P) + al(P) + al(P * 4)) + al(rts_cost_rank_log_ncc_ws_bytes(it.width, it.height));
 }
 
 bool same_items(const std::vector<rts_batch_item> &a, const rts_batch_item *b, const std::vector<int> &idx) {
@@ -265,9 +266,9 @@ int build_box_table(Chunk &c, const rts_batch_item *items, int ty, hipStream_t s
 
 int cost_pair(const rts_batch *b, const rts_batch_item &it, const PairBufs &pb, int dch, hipStream_t s) {
     const rts_pipeline_desc &d = b->base;
-    return rts_cost_rank_log_ncc_dch(pb.gray[0], pb.gray[1], pb.rank[0], pb.rank[1], pb.logimg[0], pb.logimg[1], pb.vol,
-                                     it.width, it.height, it.disp, d.ncc_hr, d.ncc_wr, d.lambda1, d.lambda2, d.lambda3,
-                                     dch, s);
+    return rts_cost_rank_log_ncc_ws(pb.gray[0], pb.gray[1], pb.rank[0], pb.rank[1], pb.logimg[0], pb.logimg[1], pb.vol,
+                                    it.width, it.height, it.disp, d.ncc_hr, d.ncc_wr, d.lambda1, d.lambda2, d.lambda3,
+                                    dch, pb.ncc_ws, rts_cost_rank_log_ncc_ws_bytes(it.width, it.height), s);
 }
 
 int prep_pair(const rts_batch *b, const rts_batch_item &it, const PairBufs &pb, hipStream_t s) {
@@ -320,6 +321,8 @@ void carve(rts_batch *b, Chunk &c, const rts_batch_item *items) {
             b.logimg[v] = (float *)p;
             p += al(P * 4);
         }
+        b.ncc_ws = p;
+        p += al(rts_cost_rank_log_ncc_ws_bytes(it.width, it.height));
     }
 }
 

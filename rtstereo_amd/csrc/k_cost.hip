@@ -496,10 +496,41 @@ __device__ __forceinline__ float ncc_fast(long N, long Sa, long Saa, long Sb, lo
     return Z > 0 ? (float)n * (float)nomN * __builtin_amdgcn_rsqf((float)n * (float)Z) : 0.0f;
 }
 
-template <int HR, int WR, int DCH>
+// Per-pixel record of one view for the row-sweep cost: the window sums of
+// the gray level and of its square over the (2HR+1) x (2WR+1) window
+// (wrapped borders, exact integers), the rank and the LoG value's bits.
+// They do not depend on the disparity, so the sweep reads them instead of
+// forming them for every disparity.
+template <int HR, int WR>
+__global__ void __launch_bounds__(256) k_ncc_records(const uint8_t *__restrict__ g, const uint8_t *__restrict__ rk,
+                                                     const float *__restrict__ lg, uint4 *__restrict__ rec, int W,
+                                                     int H) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    const bool inner = x - WR >= 0 && x + WR < W;
+    unsigned s = 0, ss = 0;
+    for (int dy = -HR; dy <= HR; dy++) {
+        const uint8_t *row = g + (size_t)wrapn(y + dy, H) * W;
+#pragma unroll
+        for (int dx = -WR; dx <= WR; dx++) {
+            const unsigned a = row[inner ? x + dx : wrapn(x + dx, W)];
+            s += a;
+            ss += a * a;
+        }
+    }
+    const size_t o = (size_t)y * W + x;
+    rec[o] = make_uint4(s, ss, rk[o], __float_as_uint(lg[o]));
+}
+
+// PRE: the gray window sums, ranks and LoG values come from the two views'
+// records (k_ncc_records), so each tap forms only the four sums with
+// products of both views; otherwise all eight sums are formed here.  The
+// integers reaching the quotient, and so the outputs, are the same.
+template <int HR, int WR, int DCH, bool PRE>
 __global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restrict__ gl, const uint8_t *__restrict__ gr,
                                                         const uint8_t *__restrict__ rkl, const uint8_t *__restrict__ rkr,
                                                         const float *__restrict__ lgl, const float *__restrict__ lgr,
+                                                        const uint4 *__restrict__ recl, const uint4 *__restrict__ recr,
                                                         float *__restrict__ vol, int W, int H, int D, float l1,
                                                         float l2, float l3) {
     constexpr int E = 2 * HR + 1, CW = 64 + 2 * WR;
@@ -544,10 +575,12 @@ __global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restric
                 for (int dx = 0; dx <= 2 * WR; dx++) {
                     const int v = buf[w][lane + dx];
                     const int a = v & 255, b = v >> 8, ab = a * b;
-                    h.sa += a;
-                    h.saa += a * a;
-                    h.sb += b;
-                    h.sbb += b * b;
+                    if constexpr (!PRE) {
+                        h.sa += a;
+                        h.saa += a * a;
+                        h.sb += b;
+                        h.sbb += b * b;
+                    }
                     h.sab += ab;
                     h.saab += a * ab;
                     h.sabb += b * ab;
@@ -557,11 +590,19 @@ __global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restric
                 ring[k] = h;
                 if (i >= 2 * HR && dok && x < W) {
                     const int yo = s0 + i - 2 * HR;
-                    const float sn = ncc_fast(N, S.sa, S.saa, S.sb, S.sbb, S.sab, S.saab, S.sabb, S.saabb);
                     const size_t o = (size_t)yo * W;
                     const int xt = wrap1(x - d, W);
-                    const float cr = fabsf((float)rkl[o + x] - (float)rkr[o + xt]);
-                    const float cl = fabsf(lgl[o + x] - lgr[o + xt]);
+                    float sn, cr, cl;
+                    if constexpr (PRE) {
+                        const uint4 ra = recl[o + x], rb = recr[o + xt];
+                        sn = ncc_fast(N, ra.x, ra.y, rb.x, rb.y, S.sab, S.saab, S.sabb, S.saabb);
+                        cr = fabsf((float)ra.z - (float)rb.z);
+                        cl = fabsf(__uint_as_float(ra.w) - __uint_as_float(rb.w));
+                    } else {
+                        sn = ncc_fast(N, S.sa, S.saa, S.sb, S.sbb, S.sab, S.saab, S.sabb, S.saabb);
+                        cr = fabsf((float)rkl[o + x] - (float)rkr[o + xt]);
+                        cl = fabsf(lgl[o + x] - lgr[o + xt]);
+                    }
                     // exp(-c/l) = exp2(c * k), k = -log2(e)/l (hardware exp2)
                     const float t1 = 1.0f - __builtin_amdgcn_exp2f(cr * k1);
                     const float t2 = 1.0f - __builtin_amdgcn_exp2f(cl * k2);
@@ -587,6 +628,34 @@ __global__ void k_combine(float *__restrict__ c1, const float *__restrict__ c2, 
 }  // namespace rts
 
 using namespace rts;
+
+static rts_status check_rln(int W, int H, int D, int hr, int wr, float l1, float l2, float l3, int dch) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(hr >= 0 && wr >= 0 && hr <= NC_RMAX && wr <= NC_RMAX, "NCC radii (%d, %d) must be in [0, %d]", hr, wr,
+              NC_RMAX);
+    RTS_CHECK(2 * hr + 1 <= H && 2 * wr + 1 <= W, "NCC window larger than the image");
+    RTS_CHECK(l1 > 0 && l2 > 0 && l3 > 0, "combine weights must be positive");
+    RTS_CHECK(dch == 8 || dch == 16 || dch == 32, "disparities per block %d must be 8, 16 or 32", dch);
+    RTS_CHECK(div_up(D, dch) <= 65535 && div_up(H, 4) <= 65535, "image too large");
+    return RTS_OK;
+}
+
+template <bool PRE>
+static void launch_rln_sweep(const uint8_t *gray_ref, const uint8_t *gray_tar, const uint8_t *rank_ref,
+                             const uint8_t *rank_tar, const float *log_ref, const float *log_tar, const uint4 *rec_ref,
+                             const uint4 *rec_tar, float *vol, int W, int H, int D, float l1, float l2, float l3,
+                             int dch, hipStream_t s) {
+    dim3 grid(div_up(W, 64), div_up(H, NS_SEG), div_up(D, dch));
+    if (dch == 8)
+        k_cost_rln_sweep<3, 3, 8, PRE><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar,
+                                                             rec_ref, rec_tar, vol, W, H, D, l1, l2, l3);
+    else if (dch == 16)
+        k_cost_rln_sweep<3, 3, 16, PRE><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar,
+                                                              rec_ref, rec_tar, vol, W, H, D, l1, l2, l3);
+    else
+        k_cost_rln_sweep<3, 3, 32, PRE><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar,
+                                                              rec_ref, rec_tar, vol, W, H, D, l1, l2, l3);
+}
 
 extern "C" {
 
@@ -694,25 +763,11 @@ rts_status rts_cost_rank_log_ncc_dch(const uint8_t *gray_ref, const uint8_t *gra
                                      const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol,
                                      int W, int H, int D, int hr, int wr, float l1, float l2, float l3, int dch,
                                      void *stream) {
-    if (int e = check_dims(W, H, D)) return e;
-    RTS_CHECK(hr >= 0 && wr >= 0 && hr <= NC_RMAX && wr <= NC_RMAX, "NCC radii (%d, %d) must be in [0, %d]", hr, wr,
-              NC_RMAX);
-    RTS_CHECK(2 * hr + 1 <= H && 2 * wr + 1 <= W, "NCC window larger than the image");
-    RTS_CHECK(l1 > 0 && l2 > 0 && l3 > 0, "combine weights must be positive");
-    RTS_CHECK(dch == 8 || dch == 16 || dch == 32, "disparities per block %d must be 8, 16 or 32", dch);
-    RTS_CHECK(div_up(D, dch) <= 65535 && div_up(H, 4) <= 65535, "image too large");
+    if (int e = check_rln(W, H, D, hr, wr, l1, l2, l3, dch)) return e;
     hipStream_t s = S(stream);
     if (hr == 3 && wr == 3) {   // the batched workload's window: row sweep
-        dim3 grid(div_up(W, 64), div_up(H, NS_SEG), div_up(D, dch));
-        if (dch == 8)
-            k_cost_rln_sweep<3, 3, 8><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol,
-                                                            W, H, D, l1, l2, l3);
-        else if (dch == 16)
-            k_cost_rln_sweep<3, 3, 16><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar,
-                                                             vol, W, H, D, l1, l2, l3);
-        else
-            k_cost_rln_sweep<3, 3, 32><<<grid, 256, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar,
-                                                             vol, W, H, D, l1, l2, l3);
+        launch_rln_sweep<false>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, nullptr, nullptr, vol, W, H,
+                                D, l1, l2, l3, dch, s);
         RTS_LAUNCHED();
         return RTS_OK;
     }
@@ -726,6 +781,34 @@ rts_status rts_cost_rank_log_ncc_dch(const uint8_t *gray_ref, const uint8_t *gra
     else
         k_cost_rank_log_ncc<32><<<grid, block, 0, s>>>(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol, W,
                                                        H, D, hr, wr, l1, l2, l3);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+size_t rts_cost_rank_log_ncc_ws_bytes(int W, int H) {
+    if (W < 1 || H < 1) return 0;
+    const size_t rec = ((size_t)W * (size_t)H * sizeof(uint4) + 255) & ~(size_t)255;
+    return 2 * rec;
+}
+
+rts_status rts_cost_rank_log_ncc_ws(const uint8_t *gray_ref, const uint8_t *gray_tar, const uint8_t *rank_ref,
+                                    const uint8_t *rank_tar, const float *log_ref, const float *log_tar, float *vol,
+                                    int W, int H, int D, int hr, int wr, float l1, float l2, float l3, int dch,
+                                    void *ws, size_t ws_bytes, void *stream) {
+    if (hr != 3 || wr != 3 || ws == nullptr)
+        return rts_cost_rank_log_ncc_dch(gray_ref, gray_tar, rank_ref, rank_tar, log_ref, log_tar, vol, W, H, D, hr,
+                                         wr, l1, l2, l3, dch, stream);
+    if (int e = check_rln(W, H, D, hr, wr, l1, l2, l3, dch)) return e;
+    const size_t need = rts_cost_rank_log_ncc_ws_bytes(W, H);
+    RTS_CHECK(ws_bytes >= need, "workspace of %zu bytes, %zu needed", ws_bytes, need);
+    RTS_CHECK(((uintptr_t)ws & 15) == 0, "workspace must be 16-byte aligned");
+    hipStream_t s = S(stream);
+    uint4 *rec_ref = (uint4 *)ws, *rec_tar = (uint4 *)((uint8_t *)ws + need / 2);
+    const dim3 rgrid(div_up(W, 64), div_up(H, 4));
+    k_ncc_records<3, 3><<<rgrid, 256, 0, s>>>(gray_ref, rank_ref, log_ref, rec_ref, W, H);
+    k_ncc_records<3, 3><<<rgrid, 256, 0, s>>>(gray_tar, rank_tar, log_tar, rec_tar, W, H);
+    launch_rln_sweep<true>(gray_ref, gray_tar, nullptr, nullptr, nullptr, nullptr, rec_ref, rec_tar, vol, W, H, D, l1,
+                           l2, l3, dch, s);
     RTS_LAUNCHED();
     return RTS_OK;
 }

@@ -270,6 +270,8 @@ struct rts_pipeline {
     uint8_t *lab[2] = {nullptr, nullptr};   // Lab images when the caller passes none
     void *aw_ws = nullptr;   // adaptive-weight band workspace
     size_t aw_ws_bytes = 0;
+    void *ncc_ws = nullptr;  // per-pixel records of the three-term cost
+    size_t ncc_ws_bytes = 0;
     // tables for the fused front end
     float *tabC = nullptr, *tabA = nullptr;
     int nA = 0;
@@ -398,6 +400,7 @@ size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
     if (uses(*d, RTS_COST_RANK)) b += 2 * P;
     if (uses(*d, RTS_COST_LOG)) b += 2 * P * 4;
     if (d->aggr == RTS_AGGR_AW) b += rts_aw_workspace_bytes(d->width, d->height, d->aw_r, d->aw_precision) + 6 * P;
+    if (d->cost3) b += rts_cost_rank_log_ncc_ws_bytes(d->width, d->height);
     return b + 8192;
 }
 
@@ -422,6 +425,10 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
         if (uses(d, RTS_COST_RANK)) A((void **)&p->rank[v], P);
         if (uses(d, RTS_COST_LOG)) A((void **)&p->logimg[v], P * 4);
         A((void **)&p->vol[v], V);
+    }
+    if (d.cost3) {
+        p->ncc_ws_bytes = rts_cost_rank_log_ncc_ws_bytes(d.width, d.height);
+        A(&p->ncc_ws, p->ncc_ws_bytes);
     }
     if (d.aggr == RTS_AGGR_AW) {
         p->aw_ws_bytes = rts_aw_workspace_bytes(d.width, d.height, d.aw_r, d.aw_precision);
@@ -475,6 +482,7 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
         (void)hipFree(p->vol[v]);
     }
     (void)hipFree(p->aw_ws);
+    (void)hipFree(p->ncc_ws);
     for (int v = 0; v < 2; v++) (void)hipFree(p->lab[v]);
     (void)hipFree(p->tabC);
     (void)hipFree(p->tabA);
@@ -659,8 +667,9 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
             if (int e = rts_convolve(gray[v], p->logimg[v], kLoG5, W, H, 2, 2, s)) return e;
         }
         RTS_MARK(p, "prep", s);
-        if (int e = rts_cost_rank_log_ncc(gray[0], gray[1], p->rank[0], p->rank[1], p->logimg[0], p->logimg[1], cur, W,
-                                          H, D, d.ncc_hr, d.ncc_wr, d.lambda1, d.lambda2, d.lambda3, s))
+        if (int e = rts_cost_rank_log_ncc_ws(gray[0], gray[1], p->rank[0], p->rank[1], p->logimg[0], p->logimg[1], cur,
+                                             W, H, D, d.ncc_hr, d.ncc_wr, d.lambda1, d.lambda2, d.lambda3, 16,
+                                             p->ncc_ws, p->ncc_ws_bytes, s))
             return e;
     } else {
         RTS_MARK(p, "prep", s);

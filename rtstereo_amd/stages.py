@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (RTS_L2R, RTS_CROSS_EXACT, RTS_AW_FP32, RTS_NCC_SEQ, call)
+from ._lib import (RTS_L2R, RTS_CROSS_EXACT, RTS_AW_FP32, RTS_NCC_SEQ, call, lib)
 from .device import DeviceArray
 
 
@@ -102,6 +102,23 @@ def ncc(ref, tar, D, hr, wr, mode=RTS_NCC_SEQ, init=None):
     if init is not None:
         v.upload(init)
     call("rts_ncc", a.ptr, b.ptr, v.ptr, W, H, D, hr, wr, mode, None)
+    return v.download()
+
+
+def rank_log_ncc(gref, gtar, rref, rtar, lref, ltar, D, hr, wr, l1, l2, l3, dch=16, ws=False):
+    """Three-term cost (rank AD, LoG AD, NCC) as a [D][H][W] volume; with
+    ``ws`` through the workspace entry point (per-pixel records)."""
+    H, W = gref.shape
+    bufs = [DeviceArray.from_host(np.ascontiguousarray(a)) for a in (gref, gtar, rref, rtar)]
+    bufs += [DeviceArray.from_host(np.ascontiguousarray(a, np.float32)) for a in (lref, ltar)]
+    v = _vol(D, H, W)
+    ptrs = [b.ptr for b in bufs]
+    if ws:
+        n = int(lib.rts_cost_rank_log_ncc_ws_bytes(W, H))
+        w = DeviceArray((n,), np.uint8)
+        call("rts_cost_rank_log_ncc_ws", *ptrs, v.ptr, W, H, D, hr, wr, l1, l2, l3, dch, w.ptr, n, None)
+    else:
+        call("rts_cost_rank_log_ncc_dch", *ptrs, v.ptr, W, H, D, hr, wr, l1, l2, l3, dch, None)
     return v.download()
 
 

@@ -22,7 +22,7 @@ def test_batch_rejects_other_configurations():
 def test_pair_bytes_and_budget():
     W, H, D = 640, 480, 64
     n = batch.pair_bytes(W, H, D)
-    assert n >= 2 * W * H * D * 4 + 12 * W * H
+    assert n >= 2 * W * H * D * 4 + 12 * W * H + 32 * W * H   # volumes, images, cost records
     b = batch.Batch(preset("rank_log_ncc", W, H, D), n - 1)
     fake = 4096   # never dereferenced: the run is refused before any device work
     it = (batch.BatchItem * 1)(batch.BatchItem(W, H, D, fake, fake, fake, fake, fake))

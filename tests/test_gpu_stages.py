@@ -94,6 +94,23 @@ def test_ncc(W, H, D, seed, hr, wr):
     assert np.all(np.abs(got - f64) <= 1e-6 * np.abs(f64) + 1e-9)
 
 
+@pytest.mark.parametrize("W,H,D,seed", [(96, 64, 24, 13), (67, 41, 13, 14), (9, 21, 5, 15), (200, 70, 40, 16)])
+def test_rank_log_ncc_records(W, H, D, seed):
+    """The workspace form of the three-term cost (per-pixel window-sum
+    records) equals the plain entry point bit for bit, for every disparity
+    chunk size; a window of another size ignores the workspace."""
+    p = _pair(W, H, D, seed)
+    gl, gr = p.gray()
+    rl, rr = S.rank(gl, 3), S.rank(gr, 3)
+    ll, lr = S.convolve(gl, oracle.LOG5, 2, 2), S.convolve(gr, oracle.LOG5, 2, 2)
+    for dch in (8, 16, 32):
+        plain = S.rank_log_ncc(gl, gr, rl, rr, ll, lr, D, 3, 3, 8.0, 10.0, 0.5, dch=dch)
+        rec = S.rank_log_ncc(gl, gr, rl, rr, ll, lr, D, 3, 3, 8.0, 10.0, 0.5, dch=dch, ws=True)
+        assert np.array_equal(plain, rec), f"dch {dch}"
+    plain = S.rank_log_ncc(gl, gr, rl, rr, ll, lr, D, 2, 1, 8.0, 10.0, 0.5)
+    assert np.array_equal(plain, S.rank_log_ncc(gl, gr, rl, rr, ll, lr, D, 2, 1, 8.0, 10.0, 0.5, ws=True))
+
+
 def test_combine():
     rng = np.random.default_rng(0)
     c1 = rng.integers(0, 63, size=(5, 17, 33)).astype(np.float32)
