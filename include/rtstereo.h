@@ -360,7 +360,8 @@ const float *rts_pipeline_volume(rts_pipeline *p);
  * on first use and replayed when the same pairs (sizes and buffers) run
  * again (a non-null stream is then required).  The first run times the
  * candidate tile shapes (cost disparities per block 8/16/32, box strip rows
- * 16/32/64) on its largest pair and keeps the fastest.  All buffers are
+ * 16/32/64) over the pairs of its first chunk and keeps the fastest (a
+ * larger one only when 2% faster).  All buffers are
  * device memory; results are complete when the stream reaches the end of
  * the run.  No reference counterpart (its driver runs one pair per
  * process, seq/Main.cpp:47-299). */

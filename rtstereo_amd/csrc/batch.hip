@@ -41,26 +41,26 @@ struct BoxPair {
 // Work item t of pair p: d = t / (ny*nx), strip = (t / nx) % ny, tile = t % nx.
 constexpr int BQ_MAXP = 256;   // pairs per chunk
 
+// A work item's position: its pair's buffers and sizes, tile origin, plane.
+struct BoxItem {
+    const float *src;
+    float *dst;
+    int W, H, x0, y0;
+    bool inner;   // the tile's window lies inside the image (no wrap)
+};
+
 template <int TY, int R>
 __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ pairs, int npairs, unsigned ntiles,
                                                    unsigned *__restrict__ counter) {
     constexpr int E = 2 * R + 1, TH = TY + 2 * R, TW = 64 + 2 * R;
+    constexpr int NB = (TH * TW + 255) / 256;
     __shared__ float tile[TH][TW + 1];
     __shared__ float hs[TH][65];
     __shared__ unsigned first[BQ_MAXP];   // each pair's first work item
     __shared__ unsigned item;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const float area = (float)(E * E);
     for (int i = threadIdx.x; i < npairs; i += 256) first[i] = pairs[i].tile0;
-    // The next work item is requested while the current one is processed.
-    unsigned nxt = 0;
-    if (threadIdx.x == 0) nxt = atomicAdd(counter, 1u);
-    for (;;) {
-        if (threadIdx.x == 0) item = nxt;
-        __syncthreads();   // also: the previous item's LDS reads are done
-        const unsigned t = item;
-        if (t >= ntiles) break;
-        if (threadIdx.x == 0) nxt = atomicAdd(counter, 1u);
+    auto decode = [&](unsigned t) {
         int lo = 0, hi = npairs - 1;   // last pair whose first item is <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -70,31 +70,65 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
         const BoxPair bp = pairs[lo];
         const unsigned u = t - bp.tile0;
         const int xb = (int)(u % bp.nx), sb = (int)((u / bp.nx) % bp.ny), d = (int)(u / ((unsigned)bp.nx * bp.ny));
-        const int W = bp.W, H = bp.H, x0 = xb * 64, y0 = sb * TY;
-        const size_t base = (size_t)d * plane(W, H);
-        const bool inner = x0 - R >= 0 && x0 + 64 + R <= W && y0 - R >= 0 && y0 + TY + R <= H;
-        const float *src = bp.in + base;
-        // All of a thread's loads are issued before its LDS stores.
-        constexpr int NB = (TH * TW + 255) / 256;
-        float v[NB];
+        const size_t base = (size_t)d * plane(bp.W, bp.H);
+        BoxItem q;
+        q.src = bp.in + base;
+        q.dst = bp.out + base;
+        q.W = bp.W;
+        q.H = bp.H;
+        q.x0 = xb * 64;
+        q.y0 = sb * TY;
+        q.inner = q.x0 - R >= 0 && q.x0 + 64 + R <= q.W && q.y0 - R >= 0 && q.y0 + TY + R <= q.H;
+        return q;
+    };
+    float v[NB];
+    auto load_tile = [&](const BoxItem &q) {
 #pragma unroll
         for (int k = 0; k < NB; k++) {
             const int i = threadIdx.x + 256 * k;
             const int rr = i / TW, cc = i - rr * TW;
             float x = 0.0f;
             if (rr < TH) {
-                const int yy = y0 - R + rr, xx = x0 - R + cc;
-                x = inner ? src[(size_t)yy * W + xx] : src[(size_t)wrapn(yy, H) * W + wrapn(xx, W)];
+                const int yy = q.y0 - R + rr, xx = q.x0 - R + cc;
+                x = q.inner ? q.src[(size_t)yy * q.W + xx] : q.src[(size_t)wrapn(yy, q.H) * q.W + wrapn(xx, q.W)];
             }
             v[k] = x;
         }
+    };
+    // Software pipeline: the next item's tile is loaded into registers
+    // while the current one is summed, and the item after that is requested
+    // from the counter one iteration ahead.
+    unsigned nxt = 0;
+    if (threadIdx.x == 0) {
+        item = atomicAdd(counter, 1u);
+        nxt = atomicAdd(counter, 1u);
+    }
+    __syncthreads();   // also: first[] is complete
+    unsigned t = item;
+    __syncthreads();   // every wave has read item before the loop rewrites it
+    BoxItem cur = {};
+    if (t < ntiles) {
+        cur = decode(t);
+        load_tile(cur);
+    }
+    while (t < ntiles) {
 #pragma unroll
         for (int k = 0; k < NB; k++) {
             const int i = threadIdx.x + 256 * k;
             const int rr = i / TW, cc = i - rr * TW;
             if (rr < TH) tile[rr][cc] = v[k];
         }
-        __syncthreads();
+        if (threadIdx.x == 0) {
+            item = nxt;
+            nxt = atomicAdd(counter, 1u);
+        }
+        __syncthreads();   // the tile is staged and the next item published
+        const unsigned tn = item;
+        BoxItem nq = {};
+        if (tn < ntiles) {
+            nq = decode(tn);
+            load_tile(nq);
+        }
         // Horizontal sums: a thread takes a 16-column piece of one row, keeps
         // its 16 + 2R inputs in registers and adds each output's E taps in
         // column order (one LDS read per input instead of E per output).
@@ -116,22 +150,25 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
         // column, its TY/4 + 2R horizontal sums in registers, taps in row
         // order.
         constexpr int RPT = TY / 4;
-        const int x = x0 + tx;
-        if (x < W) {
+        const int x = cur.x0 + tx;
+        if (x < cur.W) {
             float cv[RPT + 2 * R];
 #pragma unroll
             for (int i = 0; i < RPT + 2 * R; i++) cv[i] = hs[ty * RPT + i][tx];
 #pragma unroll
             for (int j = 0; j < RPT; j++) {
-                const int y = y0 + ty * RPT + j;
-                if (y < H) {
+                const int y = cur.y0 + ty * RPT + j;
+                if (y < cur.H) {
                     float acc = 0.0f;
 #pragma unroll
                     for (int dy = 0; dy < E; dy++) acc += cv[j + dy];
-                    stream_store(&bp.out[base + (size_t)y * W + x], __fdiv_rn(acc, area));
+                    stream_store(&cur.dst[(size_t)y * cur.W + x], div_count(acc, (uint32_t)(E * E)));
                 }
             }
         }
+        __syncthreads();   // tile, hs and item reads are done before they are rewritten
+        t = tn;
+        cur = nq;
     }
 }
 
@@ -326,37 +363,42 @@ void carve(rts_batch *b, Chunk &c, const rts_batch_item *items) {
     }
 }
 
-// Times the candidates of each tuned parameter on the largest pair of the
-// first chunk (the best of three runs after a warm-up): the box strip
-// height (16, 32, 64 rows) and the cost kernel's disparities per block (8,
-// 16, 32).  A larger candidate replaces a smaller one only when it is 2%
-// faster: on the largest pair they often tie, and the smaller ones give the
-// batch's smaller pairs more blocks.
+// Times the candidates of each tuned parameter over every pair of the first
+// chunk (the best of the timed runs after a warm-up run): the cost kernel's
+// disparities per block (8, 16, 32), one launch per pair, and the box strip
+// height (16, 32, 64 rows), one persistent launch.  The chunk's mix matters:
+// on the largest pair the candidates nearly tie, while larger tiles leave
+// the small pairs' launches with few blocks.  A larger candidate replaces a
+// smaller one only when it is 2% faster.
 int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s) {
-    size_t big = 0;
-    for (size_t i = 1; i < c.items.size(); i++)
-        if (pair_bytes(items[c.items[i]]) > pair_bytes(items[c.items[big]])) big = i;
-    Chunk one;
-    one.items = {c.items[big]};
-    one.bufs = {c.bufs[big]};
     hipEvent_t e0, e1;
     RTS_HIP(hipEventCreate(&e0));
     RTS_HIP(hipEventCreate(&e1));
-    int st = prep_pair(b, items[one.items[0]], one.bufs[0], s);
+    int st = RTS_OK;
+    for (size_t i = 0; i < c.items.size() && st == RTS_OK; i++) st = prep_pair(b, items[c.items[i]], c.bufs[i], s);
     const int dchs[3] = {8, 16, 32}, tys[3] = {16, 32, 64};
-    constexpr int REPS = 4;            // the first run warms up
+    constexpr int REPS = 3;            // the first run warms up
     constexpr float SWITCH = 0.98f;    // a larger candidate must be 2% faster
-    float best = 1e30f;
-    for (int k = 0; k < 3 && st == RTS_OK; k++) {
+    // times run(k) as the best of REPS - 1 runs after a warm-up
+    auto time_it = [&](auto run) -> float {
         float ms = 1e30f;
         for (int rep = 0; rep < REPS && st == RTS_OK; rep++) {
             float t = 0.0f;
             st = hipEventRecord(e0, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
-            if (st == RTS_OK) st = cost_pair(b, items[one.items[0]], one.bufs[0], dchs[k], s);
+            if (st == RTS_OK) st = run();
             if (st == RTS_OK) st = hipEventRecord(e1, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
             if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
             if (rep > 0 && t < ms) ms = t;
         }
+        return ms;
+    };
+    float best = 1e30f;
+    for (int k = 0; k < 3 && st == RTS_OK; k++) {
+        const float ms = time_it([&]() {
+            for (size_t i = 0; i < c.items.size(); i++)
+                if (int e = cost_pair(b, items[c.items[i]], c.bufs[i], dchs[k], s)) return e;
+            return (int)RTS_OK;
+        });
         b->tune_ms[k] = ms;
         if (k == 0 || ms < best * SWITCH) {
             best = ms;
@@ -365,24 +407,14 @@ int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s)
     }
     best = 1e30f;
     for (int k = 0; k < 3 && st == RTS_OK; k++) {
-        st = build_box_table(one, items, tys[k], s);
-        float ms = 1e30f;
-        for (int rep = 0; rep < REPS && st == RTS_OK; rep++) {
-            float t = 0.0f;
-            st = hipEventRecord(e0, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
-            if (st == RTS_OK) st = launch_box(one, tys[k], b->base.box_r, s);
-            if (st == RTS_OK) st = hipEventRecord(e1, s) == hipSuccess ? RTS_OK : fail(RTS_ERR_HIP, "event");
-            if (st == RTS_OK && hipEventSynchronize(e1) == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
-            if (rep > 0 && t < ms) ms = t;
-        }
+        st = build_box_table(c, items, tys[k], s);
+        const float ms = time_it([&]() { return launch_box(c, tys[k], b->base.box_r, s); });
         b->tune_ms[3 + k] = ms;
         if (k == 0 || ms < best * SWITCH) {
             best = ms;
             b->box_ty = tys[k];
         }
     }
-    (void)hipFree(one.dpairs);
-    (void)hipFree(one.counter);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     b->tuned = st == RTS_OK;

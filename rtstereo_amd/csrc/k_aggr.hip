@@ -72,11 +72,10 @@ __global__ void __launch_bounds__(256) k_box_rw(const float *__restrict__ in, fl
     }
     const int x = x0 + tx;
     if (x >= W) return;
-    const float area = (float)(E * E);
 #pragma unroll
     for (int i = 0; i < BOXR_RPT; i++) {
         const int y = y0 + r0 + i;
-        if (y < H) stream_store(&out[base + (size_t)y * W + x], __fdiv_rn(acc[i], area));
+        if (y < H) stream_store(&out[base + (size_t)y * W + x], div_count(acc[i], (uint32_t)(E * E)));
     }
 }
 
@@ -716,25 +715,6 @@ __device__ __forceinline__ T at_byte(const T *base, uint32_t off) {
     return *(const T *)((const char *)base + off);
 }
 
-// a / c for a support count c in [1, 2^19], rounded as IEEE division rounds
-// it.  With y = rcp(c) (relative error e, |e| <= 2^-22), q0 = a*y is within a
-// few ulp of a/c; the residual r0 = a - c*q0 is then exact (a multiple of
-// ulp(q0) no larger than 2^24 of them, since c is an integer), and
-// q0 + r0*y = a/c + (a/c - q0)*e lies within 2^-19 ulp of a/c.  A quotient a/c
-// of a float by an integer c is never a rounding midpoint in the normal range
-// and lies at least ulp/(2c) from one, so the fused q1 = RN(q0 + r0*y) is
-// RN(a/c).  Results that come out denormal, infinite or NaN (where midpoints
-// do occur, or the residual is not finite) are recomputed by the division.
-__device__ __forceinline__ float div_count(float a, uint32_t c) {
-    const float cf = (float)c;
-    const float y = __builtin_amdgcn_rcpf(cf);
-    const float q0 = a * y;
-    const float r0 = __builtin_fmaf(-q0, cf, a);
-    const float q1 = __builtin_fmaf(r0, y, q0);
-    // classes: signalling/quiet NaN, -inf, -denormal, +denormal, +inf
-    if (__builtin_expect(__builtin_amdgcn_classf(q1, 0x297), 0)) return __fdiv_rn(a, cf);
-    return q1;
-}
 
 // The sweep for arms of at most CV_LMAX (seq/CrossBasedAggregation.cpp:127-169,
 // bit-identical).  A block of NW waves owns 64 columns and SEG output rows of

@@ -58,6 +58,26 @@ __device__ __forceinline__ float sqrt_rn(float x) { return (float)__builtin_sqrt
 // so streamed volumes do not displace reusable lines from the caches.
 __device__ __forceinline__ void stream_store(float *p, float v) { __builtin_nontemporal_store(v, p); }
 
+// a / c for a count c (a window area or support size) in [1, 2^19], rounded as IEEE division rounds
+// it.  With y = rcp(c) (relative error e, |e| <= 2^-22), q0 = a*y is within a
+// few ulp of a/c; the residual r0 = a - c*q0 is then exact (a multiple of
+// ulp(q0) no larger than 2^24 of them, since c is an integer), and
+// q0 + r0*y = a/c + (a/c - q0)*e lies within 2^-19 ulp of a/c.  A quotient a/c
+// of a float by an integer c is never a rounding midpoint in the normal range
+// and lies at least ulp/(2c) from one, so the fused q1 = RN(q0 + r0*y) is
+// RN(a/c).  Results that come out denormal, infinite or NaN (where midpoints
+// do occur, or the residual is not finite) are recomputed by the division.
+__device__ __forceinline__ float div_count(float a, uint32_t c) {
+    const float cf = (float)c;
+    const float y = __builtin_amdgcn_rcpf(cf);
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-q0, cf, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    // classes: signalling/quiet NaN, -inf, -denormal, +denormal, +inf
+    if (__builtin_expect(__builtin_amdgcn_classf(q1, 0x297), 0)) return __fdiv_rn(a, cf);
+    return q1;
+}
+
 // Word 3 of a raw buffer resource on CDNA (gfx9 family): 32-bit data format,
 // no swizzle, no stride, so an access is base + soffset + voffset.
 constexpr int BUF_DWORD3 = 0x00020000;
