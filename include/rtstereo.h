@@ -360,7 +360,7 @@ const float *rts_pipeline_volume(rts_pipeline *p);
  * on first use and replayed when the same pairs (sizes and buffers) run
  * again (a non-null stream is then required).  The first run times the
  * candidate tile shapes (cost disparities per block 8/16/32, box strip rows
- * 16/32/64) over the pairs of its first chunk and keeps the fastest (a
+ * 32, 64-2r rounded down to a multiple of 4, 64) over the pairs of its first chunk and keeps the fastest (a
  * larger one only when 2% faster).  All buffers are
  * device memory; results are complete when the stream reaches the end of
  * the run.  No reference counterpart (its driver runs one pair per
@@ -376,7 +376,8 @@ typedef struct rts_batch_stats {
     unsigned long long box_items;           /* cumulative box work items */
     size_t arena_bytes;
     int box_rows, cost_disparities;         /* tuned tile shape */
-    float tune_ms[6];                       /* cost dch 8/16/32, box rows 16/32/64 */
+    float tune_ms[6];                       /* cost dch 8/16/32, box rows 32 / 64-2r rounded
+                                               down to a multiple of 4 / 64 */
 } rts_batch_stats;
 typedef struct rts_batch rts_batch;
 rts_status rts_batch_create(const rts_pipeline_desc *base, size_t budget_bytes, int use_graphs, rts_batch **out);

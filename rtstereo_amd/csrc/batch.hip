@@ -56,18 +56,18 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
     constexpr int NB = (TH * TW + 255) / 256;
     __shared__ float tile[TH][TW + 1];
     __shared__ float hs[TH][65];
-    __shared__ unsigned first[BQ_MAXP];   // each pair's first work item
+    __shared__ BoxPair sp[BQ_MAXP];   // the pair table: decoding an item reads no global memory
     __shared__ unsigned item;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < npairs; i += 256) first[i] = pairs[i].tile0;
+    for (int i = threadIdx.x; i < npairs; i += 256) sp[i] = pairs[i];
     auto decode = [&](unsigned t) {
         int lo = 0, hi = npairs - 1;   // last pair whose first item is <= t
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (first[mid] <= t) lo = mid;
+            if (sp[mid].tile0 <= t) lo = mid;
             else hi = mid - 1;
         }
-        const BoxPair bp = pairs[lo];
+        const BoxPair bp = sp[lo];
         const unsigned u = t - bp.tile0;
         const int xb = (int)(u % bp.nx), sb = (int)((u / bp.nx) % bp.ny), d = (int)(u / ((unsigned)bp.nx * bp.ny));
         const size_t base = (size_t)d * plane(bp.W, bp.H);
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
         item = atomicAdd(counter, 1u);
         nxt = atomicAdd(counter, 1u);
     }
-    __syncthreads();   // also: first[] is complete
+    __syncthreads();   // also: the pair table is complete
     unsigned t = item;
     __syncthreads();   // every wave has read item before the loop rewrites it
     BoxItem cur = {};
@@ -239,13 +239,20 @@ void free_chunk(Chunk *c) {
     c->counter = nullptr;
 }
 
+// The strip height whose staged rows (TY + 2R) split into at most 256
+// sixteen-column pieces, so the horizontal pass is one round of the block:
+// 64 - 2R rounded down to a multiple of 4 (52 rows for the 11x11 box).
+constexpr int box_balanced_rows(int r) { return ((64 - 2 * r) / 4) * 4; }
+
 template <int R>
 void launch_box_r(const Chunk &c, int ty, int nb, hipStream_t s) {
-    switch (ty) {
-        case 16: k_box_queue<16, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter); break;
-        case 64: k_box_queue<64, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter); break;
-        default: k_box_queue<32, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter); break;
-    }
+    constexpr int TB = box_balanced_rows(R);
+    if (ty == 32)
+        k_box_queue<32, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
+    else if (ty == TB)
+        k_box_queue<TB, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
+    else
+        k_box_queue<64, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
 }
 
 int launch_box(const Chunk &c, int ty, int r, hipStream_t s) {
@@ -366,7 +373,8 @@ void carve(rts_batch *b, Chunk &c, const rts_batch_item *items) {
 // Times the candidates of each tuned parameter over every pair of the first
 // chunk (the best of the timed runs after a warm-up run): the cost kernel's
 // disparities per block (8, 16, 32), one launch per pair, and the box strip
-// height (16, 32, 64 rows), one persistent launch.  The chunk's mix matters:
+// height (32 rows, the balanced height of box_balanced_rows, 64 rows), one
+// persistent launch.  The chunk's mix matters:
 // on the largest pair the candidates nearly tie, while larger tiles leave
 // the small pairs' launches with few blocks.  A larger candidate replaces a
 // smaller one only when it is 2% faster.
@@ -376,7 +384,7 @@ int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s)
     RTS_HIP(hipEventCreate(&e1));
     int st = RTS_OK;
     for (size_t i = 0; i < c.items.size() && st == RTS_OK; i++) st = prep_pair(b, items[c.items[i]], c.bufs[i], s);
-    const int dchs[3] = {8, 16, 32}, tys[3] = {16, 32, 64};
+    const int dchs[3] = {8, 16, 32}, tys[3] = {32, box_balanced_rows(b->base.box_r), 64};
     constexpr int REPS = 3;            // the first run warms up
     constexpr float SWITCH = 0.98f;    // a larger candidate must be 2% faster
     // times run(k) as the best of REPS - 1 runs after a warm-up
