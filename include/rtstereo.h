@@ -338,8 +338,7 @@ rts_status rts_pipeline_stage_ms(rts_pipeline *p, float *ms, const char **names,
  * and stream pointers captures the run's launch sequence into a HIP graph
  * and every later run with those pointers replays it (new contents in the
  * same buffers are read as usual).  Runs with other pointers, on the null
- * stream, with timing enabled or with the literal WTA carry (wta_literal,
- * which allocates scratch per run) launch their kernels one by one.  For
+ * stream or with timing enabled launch their kernels one by one.  For
  * census + cross pipelines the captured graph runs the right view's
  * preparation (gray, packed colour, arms, census) beside the left view's,
  * on a second stream that this call creates.  No reference counterpart (the

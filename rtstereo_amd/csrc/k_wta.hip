@@ -34,8 +34,7 @@ __device__ __forceinline__ bool beats(float v, float best) {
 
 template <int PPT, bool MAXIMIZE, bool WANT_R, int DU, int T>
 __global__ void __launch_bounds__(T) k_wta(const float *__restrict__ vol, uint8_t *__restrict__ dl,
-                                                     uint8_t *__restrict__ dr, uint8_t *__restrict__ beat0, int W,
-                                                     int H, int D) {
+                                                     uint8_t *__restrict__ dr, int W, int H, int D) {
     // Chunks of a row are consecutive in the work order and each XCD takes
     // one contiguous range of it, so the right view's reads past the chunk
     // (columns up to x + D - 1) find the next chunk's lines in the same L2.
@@ -96,51 +95,84 @@ __global__ void __launch_bounds__(T) k_wta(const float *__restrict__ vol, uint8_
         const int x = xb + i * T;
         if (x < W) {
             dl[rowoff + x] = (uint8_t)dL[i];
-            if (beat0) beat0[rowoff + x] = dL[i] != 0;
             if (WANT_R) dr[rowoff + x] = (uint8_t)dR[i];
         }
     }
 }
 
-// Literal WTA carry (seq/WTA.cpp:12-15 never resets `disp`): a pixel whose
-// d=0 cost is not beaten takes the disparity of the last earlier pixel in
-// raster order that was beaten (0 if none).  Pass 1: per row, the last
-// beaten disparity (or -1).  Pass 2 (one thread): carry into each row.
-// Pass 3: per row, a sequential fill.
-__global__ void k_carry_rowlast(const uint8_t *__restrict__ dl, const uint8_t *__restrict__ beat, int *__restrict__ last,
-                                int W, int H) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= H) return;
+// Literal WTA carry (seq/WTA.cpp:12-15 never resets `disp`): a pixel keeps
+// the previous pixel's disparity (raster order) unless some d > 0 beats the
+// running best, which starts at d = 0.  With the lowest d winning ties, a
+// pixel is beaten exactly when its own argbest is not 0, so the literal map
+// is: the argbest where it is not 0, else the last non-zero argbest before
+// the pixel in raster order (0 if there is none).  Three passes over the
+// finished map, with one int of scratch per row:
+//   1. per row, the last non-zero disparity (-1 if none);
+//   2. one block turns those into the carry into each row (an exclusive
+//      "latest non-negative" scan over rows);
+//   3. per row, zeros take the carry, or the last non-zero left of them.
+// Rows are one wave each; lane j walks the row segment [j*c, (j+1)*c).
+__device__ __forceinline__ int seg_last_nonzero(const uint8_t *row, int x0, int x1) {
     int v = -1;
-    for (int x = W - 1; x >= 0; x--)
-        if (beat[(size_t)y * W + x]) {
-            v = dl[(size_t)y * W + x];
-            break;
-        }
-    last[y] = v;
+    for (int x = x0; x < x1; x++) {
+        const int t = row[x];
+        if (t) v = t;
+    }
+    return v;
 }
 
-__global__ void k_carry_scan(int *__restrict__ last, int H) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int carry = 0;
-    for (int y = 0; y < H; y++) {
-        const int v = last[y];
-        last[y] = carry;   // carry into row y
-        if (v >= 0) carry = v;
+__global__ void __launch_bounds__(64) k_carry_rowlast(const uint8_t *__restrict__ dl, int *__restrict__ last, int W) {
+    const int y = blockIdx.x, lane = threadIdx.x;
+    const int c = (W + 63) / 64, x0 = min(lane * c, W), x1 = min(x0 + c, W);
+    const int v = seg_last_nonzero(dl + (size_t)y * W, x0, x1);
+    const uint64_t m = __ballot(v >= 0);
+    const int src = m ? 63 - __clzll((long long)m) : 0;
+    const int r = __shfl(v, src);
+    if (lane == 0) last[y] = m ? r : -1;
+}
+
+// carry[y] <- the last last[j] >= 0 with j < y, else 0 (in place).
+__global__ void __launch_bounds__(1024) k_carry_scan(int *__restrict__ carry, int H) {
+    __shared__ int key[1024], val[1024];
+    const int t = threadIdx.x;
+    int in = 0;   // carried into the current chunk of rows
+    for (int base = 0; base < H; base += 1024) {
+        const int i = base + t;
+        const int v = i < H ? carry[i] : -1;
+        val[t] = v;
+        key[t] = v >= 0 ? t : -1;
+        __syncthreads();
+        // inclusive max-scan of key: the latest row <= t of the chunk with a value
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int k = t >= off ? key[t - off] : -1;
+            __syncthreads();
+            if (k > key[t]) key[t] = k;
+            __syncthreads();
+        }
+        const int kp = t > 0 ? key[t - 1] : -1;
+        if (i < H) carry[i] = kp >= 0 ? val[kp] : in;
+        const int kl = key[1023];
+        in = kl >= 0 ? val[kl] : in;
+        __syncthreads();
     }
 }
 
-__global__ void k_carry_fill(uint8_t *__restrict__ dl, const uint8_t *__restrict__ beat, const int *__restrict__ carry_in,
-                             int W, int H) {
-    const int y = blockIdx.x * blockDim.x + threadIdx.x;
-    if (y >= H) return;
-    int c = carry_in[y];
-    for (int x = 0; x < W; x++) {
-        const size_t p = (size_t)y * W + x;
-        if (beat[p])
-            c = dl[p];
+__global__ void __launch_bounds__(64) k_carry_fill(uint8_t *__restrict__ dl, const int *__restrict__ carry, int W) {
+    const int y = blockIdx.x, lane = threadIdx.x;
+    const int c = (W + 63) / 64, x0 = min(lane * c, W), x1 = min(x0 + c, W);
+    uint8_t *row = dl + (size_t)y * W;
+    const int v = seg_last_nonzero(row, x0, x1);
+    // the latest lower lane with a non-zero disparity, else the row's carry
+    const uint64_t m = __ballot(v >= 0) & ((1ull << lane) - 1ull);
+    const int src = m ? 63 - __clzll((long long)m) : 0;
+    const int r = __shfl(v, src);
+    int cur = m ? r : carry[y];
+    for (int x = x0; x < x1; x++) {
+        const int t = row[x];
+        if (t)
+            cur = t;
         else
-            dl[p] = (uint8_t)c;
+            row[x] = (uint8_t)cur;
     }
 }
 
@@ -159,9 +191,30 @@ __global__ void k_cross_check(const uint8_t *__restrict__ dl, const uint8_t *__r
 // seq/CrossBasedAggregation.cpp:171-212 out of place: histogram of the
 // disparities in rows y-top .. y+bottom-1 (exclusive end, as the reference),
 // columns x-left .. x+right of each row's own arms; most frequent, lowest on
-// ties.  Histogram in registers-by-LDS: one thread per pixel with a private
-// 256-bin byte-packed counter slice would not fit, so bins live in LDS per
-// thread group of 64 pixels x D bins (D <= 256, counts <= 35*35).
+// ties.  One thread per pixel, 64 pixels per block, 16-bit bins in LDS
+// ([256][64]).  A support of more than 65535 pixels (only possible with arms
+// above 127) could wrap a 16-bit bin, so such a pixel counts again with a
+// 32-bit counter per disparity value.
+__device__ int refine_mode_wide(const uint8_t *__restrict__ disp, const uint32_t *__restrict__ arms, int x, int y,
+                                int W, int H, int D) {
+    const uint32_t a = arms[(size_t)y * W + x];
+    int best = 0;
+    uint32_t bv = 0;
+    for (int v = 0; v < D; v++) {
+        uint32_t c = 0;
+        for (int row = max(y - arm_t(a), 0); row < min(y + arm_b(a), H); row++) {
+            const uint32_t ar = arms[(size_t)row * W + x];
+            const int xl = max(x - arm_l(ar), 0), xr = min(x + arm_r(ar), W - 1);
+            for (int xx = xl; xx <= xr; xx++) c += disp[(size_t)row * W + xx] == v;
+        }
+        if (c > bv || v == 0) {
+            bv = c;
+            best = v;
+        }
+    }
+    return best;
+}
+
 __global__ void __launch_bounds__(64) k_disp_refine(const uint8_t *__restrict__ disp, uint8_t *__restrict__ out,
                                                     const uint32_t *__restrict__ arms, int W, int H, int D) {
     __shared__ uint16_t hist[256][64];
@@ -169,12 +222,14 @@ __global__ void __launch_bounds__(64) k_disp_refine(const uint8_t *__restrict__ 
     for (int i = 0; i < D; i++) hist[i][threadIdx.x] = 0;
     if (x >= W) return;
     const uint32_t a = arms[(size_t)y * W + x];
+    uint32_t tot = 0;
     for (int row = y - arm_t(a); row < y + arm_b(a); row++) {
         if (row < 0 || row >= H) continue;
         const uint32_t ar = arms[(size_t)row * W + x];
         int xl = x - arm_l(ar), xr = x + arm_r(ar);
         if (xl < 0) xl = 0;
         if (xr >= W) xr = W - 1;
+        tot += (uint32_t)(xr - xl + 1);
         for (int xx = xl; xx <= xr; xx++) {
             const int v = disp[(size_t)row * W + xx];
             if (v < D) hist[v][threadIdx.x]++;
@@ -188,38 +243,110 @@ __global__ void __launch_bounds__(64) k_disp_refine(const uint8_t *__restrict__ 
             best = i;
         }
     }
+    if (tot > 65535u) best = refine_mode_wide(disp, arms, x, y, W, H, D);
     out[(size_t)y * W + x] = (uint8_t)best;
 }
 
-// Literal (in-place, raster order) refinement: inherently sequential; one
-// thread walks the image.  Only for parity with the reference's in-place
-// update on small inputs.
-__global__ void k_disp_refine_literal(uint8_t *__restrict__ disp, const uint32_t *__restrict__ arms, int W, int H, int D,
-                                      int *__restrict__ hist) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int y = 0; y < H; y++)
-        for (int x = 0; x < W; x++) {
-            for (int i = 0; i < D; i++) hist[i] = 0;
+// Literal refinement: the reference updates dispIm in place in raster order
+// (seq/CrossBasedAggregation.cpp:175-211), so a pixel's vote sees the refined
+// values of the pixels before it and the original values of those after it.
+// Pixel (y, x) reads only columns within A = the largest left or right arm of
+// x.  With K = A + 1, every pixel it reads in an earlier row lies on an
+// earlier anti-diagonal s = x + K*y, every pixel in a later row on a later
+// one, and two pixels of one diagonal on different rows are at least K
+// columns apart, so they never read each other.  One block therefore walks
+// the diagonals in order with a barrier between them; its waves refine a
+// diagonal's pixels in parallel, one wave per pixel with the histogram in
+// LDS (32-bit bins).  A wave first loads the arms of all its support rows
+// (lane = row), then the rows' disparities eight rows at a time (lanes over
+// columns), so that their loads are in flight together.
+constexpr int RF_WAVES = 16, RF_UNROLL = 8;
+
+__global__ void __launch_bounds__(64 * RF_WAVES) k_disp_refine_diag(uint8_t *__restrict__ disp,
+                                                                    const uint32_t *__restrict__ arms, int W, int H,
+                                                                    int D) {
+    __shared__ uint32_t hist[RF_WAVES][256];
+    __shared__ uint32_t wmax[RF_WAVES];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // A: the largest horizontal arm of the image
+    uint32_t am = 0;
+    for (size_t i = threadIdx.x; i < (size_t)W * H; i += blockDim.x) {
+        const uint32_t a = arms[i];
+        am = max(am, max(a & 255u, (a >> 8) & 255u));
+    }
+    for (int off = 32; off >= 1; off >>= 1) am = max(am, (uint32_t)__shfl_xor((int)am, off));
+    if (lane == 0) wmax[w] = am;
+    __syncthreads();
+    for (int i = 0; i < RF_WAVES; i++) am = max(am, wmax[i]);
+    const long long K = (long long)am + 1;
+    const long long T = (long long)(W - 1) + K * (H - 1);
+    uint32_t *hw = hist[w];
+    for (long long t = 0; t <= T; t++) {
+        const int ylo = t >= W ? (int)((t - W + K) / K) : 0;
+        const int yhi = (int)min((long long)(H - 1), t / K);
+        for (int y = ylo + w; y <= yhi; y += RF_WAVES) {
+            const int x = (int)(t - K * y);
+            for (int i = lane; i < 256; i += 64) hw[i] = 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const uint32_t a = arms[(size_t)y * W + x];
-            for (int row = y - arm_t(a); row < y + arm_b(a); row++) {
-                if (row < 0 || row >= H) continue;
-                const uint32_t ar = arms[(size_t)row * W + x];
-                int xl = x - arm_l(ar), xr = x + arm_r(ar);
-                if (xl < 0) xl = 0;
-                if (xr >= W) xr = W - 1;
-                for (int xx = xl; xx <= xr; xx++) {
-                    const int v = disp[(size_t)row * W + xx];
-                    if (v < D) hist[v]++;
+            const int r0 = max(y - arm_t(a), 0), r1 = min(y + arm_b(a), H);
+            for (int jb = r0; jb < r1; jb += 64) {
+                const int nb = min(64, r1 - jb);
+                // lane j: the extent of support row jb + j
+                int xl = 0, n = 0;
+                if (lane < nb) {
+                    const uint32_t ar = arms[(size_t)(jb + lane) * W + x];
+                    xl = max(x - arm_l(ar), 0);
+                    n = min(x + arm_r(ar), W - 1) - xl + 1;
+                }
+                for (int k0 = 0; k0 < nb; k0 += RF_UNROLL) {
+                    int v[RF_UNROLL];
+#pragma unroll
+                    for (int u = 0; u < RF_UNROLL; u++) {
+                        const int k = min(k0 + u, nb - 1);
+                        const int rxl = __shfl(xl, k), rn = __shfl(n, k);
+                        v[u] = k0 + u < nb && lane < rn ? disp[(size_t)(jb + k) * W + rxl + lane] : 256;
+                    }
+#pragma unroll
+                    for (int u = 0; u < RF_UNROLL; u++)
+                        if (v[u] < D) atomicAdd(&hw[v[u]], 1u);
+                    // rows wider than 64 columns (arms above 31)
+                    for (int u = 0; u < RF_UNROLL && k0 + u < nb; u++) {
+                        const int k = k0 + u;
+                        const int rxl = __shfl(xl, k), rn = __shfl(n, k);
+                        for (int c = 64 + lane; c < rn; c += 64) {
+                            const int vv = disp[(size_t)(jb + k) * W + rxl + c];
+                            if (vv < D) atomicAdd(&hw[vv], 1u);
+                        }
+                    }
                 }
             }
-            int best = 0, bv = hist[0];
-            for (int i = 1; i < D; i++)
-                if (hist[i] > bv) {
-                    bv = hist[i];
-                    best = i;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // most frequent value, lowest on ties: each lane its four bins in
+            // ascending order, then the wave's best (count, then index)
+            uint32_t bc = 0;
+            int bi = 256;
+            for (int i = lane; i < D; i += 64)
+                if (bi == 256 || hw[i] > bc) {
+                    bc = hw[i];
+                    bi = i;
                 }
-            disp[(size_t)y * W + x] = (uint8_t)best;
+            for (int off = 32; off >= 1; off >>= 1) {
+                const uint32_t oc = (uint32_t)__shfl_xor((int)bc, off);
+                const int oi = __shfl_xor(bi, off);
+                if (oi != 256 && (bi == 256 || oc > bc || (oc == bc && oi < bi))) {
+                    bc = oc;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) disp[(size_t)y * W + x] = (uint8_t)bi;
         }
+        __syncthreads();
+    }
 }
 
 // Rows are cut into chunks, one block each.  One wave per block and 8 pixels per lane (512-pixel row chunks): a 1080p
@@ -230,34 +357,55 @@ __global__ void k_disp_refine_literal(uint8_t *__restrict__ disp, const uint32_t
 constexpr int WTA_T = 64;
 
 template <int PPT, bool MAXIMIZE, bool WANT_R>
-static void wta_go(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *beat, int W, int H, int D, hipStream_t s) {
+static void wta_go(const float *vol, uint8_t *dl, uint8_t *dr, int W, int H, int D, hipStream_t s) {
     constexpr int DU = PPT >= 8 ? 1 : 4;
-    k_wta<PPT, MAXIMIZE, WANT_R, DU, WTA_T><<<dim3(div_up(W, WTA_T * PPT), H), WTA_T, 0, s>>>(vol, dl, dr, beat, W, H,
-                                                                                             D);
+    k_wta<PPT, MAXIMIZE, WANT_R, DU, WTA_T><<<dim3(div_up(W, WTA_T * PPT), H), WTA_T, 0, s>>>(vol, dl, dr, W, H, D);
 }
 
 template <bool MAXIMIZE, bool WANT_R>
-static void launch_wta_r(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *beat, int W, int H, int D,
-                         hipStream_t s) {
+static void launch_wta_r(const float *vol, uint8_t *dl, uint8_t *dr, int W, int H, int D, hipStream_t s) {
     if (W <= 64)
-        wta_go<1, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+        wta_go<1, MAXIMIZE, WANT_R>(vol, dl, dr, W, H, D, s);
     else if (W <= 128)
-        wta_go<2, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+        wta_go<2, MAXIMIZE, WANT_R>(vol, dl, dr, W, H, D, s);
     else if (W <= 256)
-        wta_go<4, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+        wta_go<4, MAXIMIZE, WANT_R>(vol, dl, dr, W, H, D, s);
     else
-        wta_go<8, MAXIMIZE, WANT_R>(vol, dl, dr, beat, W, H, D, s);
+        wta_go<8, MAXIMIZE, WANT_R>(vol, dl, dr, W, H, D, s);
 }
 
 template <bool MAXIMIZE>
-static int launch_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *beat, int W, int H, int D,
-                      hipStream_t s) {
-    RTS_CHECK(H <= 65535, "image height %d too large", H);
+static void launch_wta(const float *vol, uint8_t *dl, uint8_t *dr, int W, int H, int D, hipStream_t s) {
     if (dr)
-        launch_wta_r<MAXIMIZE, true>(vol, dl, dr, beat, W, H, D, s);
+        launch_wta_r<MAXIMIZE, true>(vol, dl, dr, W, H, D, s);
     else
-        launch_wta_r<MAXIMIZE, false>(vol, dl, dr, beat, W, H, D, s);
+        launch_wta_r<MAXIMIZE, false>(vol, dl, dr, W, H, D, s);
+}
+
+int wta_launch(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags, int tol,
+               int *carry, hipStream_t s) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(dl != nullptr, "rts_wta needs dl");
+    RTS_CHECK(mask == nullptr || dr != nullptr, "the left-right mask needs dr");
+    RTS_CHECK(H <= WTA_MAX_ROWS, "image height %d too large (at most %d rows)", H, WTA_MAX_ROWS);
+    const bool literal = flags & RTS_WTA_LITERAL;
+    RTS_CHECK(!literal || carry != nullptr, "the literal carry needs its per-row scratch");
+    if (flags & RTS_WTA_MAX)
+        launch_wta<true>(vol, dl, dr, W, H, D, s);
+    else
+        launch_wta<false>(vol, dl, dr, W, H, D, s);
     RTS_LAUNCHED();
+    if (literal) {
+        k_carry_rowlast<<<H, 64, 0, s>>>(dl, carry, W);
+        k_carry_scan<<<1, 1024, 0, s>>>(carry, H);
+        k_carry_fill<<<H, 64, 0, s>>>(dl, carry, W);
+        RTS_LAUNCHED();
+    }
+    // the left-right check runs on the final D_L (after the literal carry)
+    if (mask) {
+        k_cross_check<<<dim3(div_up(W, 256), H), 256, 0, s>>>(dl, dr, mask, W, H, tol);
+        RTS_LAUNCHED();
+    }
     return RTS_OK;
 }
 
@@ -269,35 +417,12 @@ extern "C" {
 
 rts_status rts_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags, int tol,
                    void *stream) {
-    if (int e = check_dims(W, H, D)) return e;
-    RTS_CHECK(dl != nullptr, "rts_wta needs dl");
-    RTS_CHECK(mask == nullptr || dr != nullptr, "the left-right mask needs dr");
-    const bool maximize = flags & RTS_WTA_MAX;
-    const bool literal = flags & RTS_WTA_LITERAL;
     hipStream_t s = S(stream);
-    uint8_t *beat = nullptr;
-    int *last = nullptr;
-    if (literal) {
-        // Scratch for the carry passes (per-pixel flags and per-row carries).
-        RTS_HIP(hipMallocAsync((void **)&beat, (size_t)W * H, s));
-        RTS_HIP(hipMallocAsync((void **)&last, sizeof(int) * (size_t)H, s));
-    }
-    int st = maximize ? launch_wta<true>(vol, dl, dr, beat, W, H, D, s) : launch_wta<false>(vol, dl, dr, beat, W, H, D, s);
-    if (st == RTS_OK && literal) {
-        k_carry_rowlast<<<div_up(H, 64), 64, 0, s>>>(dl, beat, last, W, H);
-        k_carry_scan<<<1, 1, 0, s>>>(last, H);
-        k_carry_fill<<<div_up(H, 64), 64, 0, s>>>(dl, beat, last, W, H);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) st = fail(RTS_ERR_HIP, "carry launch: %s", hipGetErrorString(e));
-    }
-    // the left-right check runs on the final D_L (after the literal carry)
-    if (st == RTS_OK && mask) {
-        k_cross_check<<<dim3(div_up(W, 256), H), 256, 0, s>>>(dl, dr, mask, W, H, tol);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) st = fail(RTS_ERR_HIP, "cross-check launch: %s", hipGetErrorString(e));
-    }
-    if (beat) (void)hipFreeAsync(beat, s);
-    if (last) (void)hipFreeAsync(last, s);
+    int *carry = nullptr;
+    if ((flags & RTS_WTA_LITERAL) && H >= 1 && H <= WTA_MAX_ROWS)
+        RTS_HIP(hipMallocAsync((void **)&carry, sizeof(int) * (size_t)H, s));
+    const int st = wta_launch(vol, dl, dr, mask, W, H, D, flags, tol, carry, s);
+    if (carry) (void)hipFreeAsync(carry, s);
     return st;
 }
 
@@ -313,12 +438,8 @@ rts_status rts_disp_refine(uint8_t *disp, uint8_t *out, const uint32_t *arms, in
     if (int e = check_dims(W, H, D)) return e;
     hipStream_t s = S(stream);
     if (literal) {
-        int *hist = nullptr;
-        RTS_HIP(hipMallocAsync((void **)&hist, sizeof(int) * 256, s));
-        k_disp_refine_literal<<<1, 1, 0, s>>>(disp, arms, W, H, D, hist);
-        hipError_t e = hipGetLastError();
-        (void)hipFreeAsync(hist, s);
-        if (e != hipSuccess) return fail(RTS_ERR_HIP, "refine launch: %s", hipGetErrorString(e));
+        k_disp_refine_diag<<<1, 64 * RF_WAVES, 0, s>>>(disp, arms, W, H, D);
+        RTS_LAUNCHED();
         return RTS_OK;
     }
     RTS_CHECK(out != nullptr && out != disp, "out-of-place refinement needs a separate output");

@@ -272,6 +272,7 @@ struct rts_pipeline {
     size_t aw_ws_bytes = 0;
     void *ncc_ws = nullptr;  // per-pixel records of the three-term cost
     size_t ncc_ws_bytes = 0;
+    int *carry = nullptr;    // per-row scratch of the literal WTA carry
     // tables for the fused front end
     float *tabC = nullptr, *tabA = nullptr;
     int nA = 0;
@@ -401,6 +402,7 @@ size_t rts_pipeline_bytes(const rts_pipeline_desc *d) {
     if (uses(*d, RTS_COST_LOG)) b += 2 * P * 4;
     if (d->aggr == RTS_AGGR_AW) b += rts_aw_workspace_bytes(d->width, d->height, d->aw_r, d->aw_precision) + 6 * P;
     if (d->cost3) b += rts_cost_rank_log_ncc_ws_bytes(d->width, d->height);
+    if (d->wta_literal) b += sizeof(int) * (size_t)d->height;
     return b + 8192;
 }
 
@@ -430,6 +432,7 @@ rts_status rts_pipeline_create(const rts_pipeline_desc *desc, rts_pipeline **out
         p->ncc_ws_bytes = rts_cost_rank_log_ncc_ws_bytes(d.width, d.height);
         A(&p->ncc_ws, p->ncc_ws_bytes);
     }
+    if (d.wta_literal) A((void **)&p->carry, sizeof(int) * (size_t)d.height);
     if (d.aggr == RTS_AGGR_AW) {
         p->aw_ws_bytes = rts_aw_workspace_bytes(d.width, d.height, d.aw_r, d.aw_precision);
         A(&p->aw_ws, p->aw_ws_bytes);
@@ -483,6 +486,7 @@ rts_status rts_pipeline_destroy(rts_pipeline *p) {
     }
     (void)hipFree(p->aw_ws);
     (void)hipFree(p->ncc_ws);
+    (void)hipFree(p->carry);
     for (int v = 0; v < 2; v++) (void)hipFree(p->lab[v]);
     (void)hipFree(p->tabC);
     (void)hipFree(p->tabA);
@@ -711,7 +715,8 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
     RTS_MARK(p, "aggr", s);
     p->final_vol = cur;
     int flags = (d.cost1 == RTS_COST_NCC ? RTS_WTA_MAX : 0) | (d.wta_literal ? RTS_WTA_LITERAL : 0);
-    if (int e = rts_wta(cur, dl, d.lr_check ? dr : nullptr, d.lr_check ? mask : nullptr, W, H, D, flags, d.lr_tol, s))
+    if (int e = wta_launch(cur, dl, d.lr_check ? dr : nullptr, d.lr_check ? mask : nullptr, W, H, D, flags, d.lr_tol,
+                           p->carry, s))
         return e;
     RTS_MARK(p, "wta", s);
     return RTS_OK;
@@ -722,9 +727,7 @@ extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb,
                                        const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask,
                                        void *stream) {
     RTS_CHECK(p != nullptr, "null pipeline");
-    // the literal WTA carry allocates its scratch on the stream per run, so
-    // those pipelines are never captured
-    if (!p->graphs || p->timing || stream == nullptr || p->d.wta_literal)
+    if (!p->graphs || p->timing || stream == nullptr)
         return pipeline_launch(p, left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,
                                stream);
     const rts_pipeline::RunKey key{{left_rgb, right_rgb, left_gray, right_gray, left_lab, right_lab, dl, dr, mask,

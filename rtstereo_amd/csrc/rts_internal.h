@@ -66,15 +66,17 @@ __device__ __forceinline__ void stream_store(float *p, float v) { __builtin_nont
 // of a float by an integer c is never a rounding midpoint in the normal range
 // and lies at least ulp/(2c) from one, so the fused q1 = RN(q0 + r0*y) is
 // RN(a/c).  Results that come out denormal, infinite or NaN (where midpoints
-// do occur, or the residual is not finite) are recomputed by the division.
+// do occur, or the residual is not finite) are recomputed by the division,
+// and so are zero results: for a = -0 the fused steps give +0 where the
+// division gives -0.
 __device__ __forceinline__ float div_count(float a, uint32_t c) {
     const float cf = (float)c;
     const float y = __builtin_amdgcn_rcpf(cf);
     const float q0 = a * y;
     const float r0 = __builtin_fmaf(-q0, cf, a);
     const float q1 = __builtin_fmaf(r0, y, q0);
-    // classes: signalling/quiet NaN, -inf, -denormal, +denormal, +inf
-    if (__builtin_expect(__builtin_amdgcn_classf(q1, 0x297), 0)) return __fdiv_rn(a, cf);
+    // classes: signalling/quiet NaN, -inf, -denormal, -0, +0, +denormal, +inf
+    if (__builtin_expect(__builtin_amdgcn_classf(q1, 0x2F7), 0)) return __fdiv_rn(a, cf);
     return q1;
 }
 
@@ -98,6 +100,16 @@ __host__ __device__ __forceinline__ size_t plane(int W, int H) { return (size_t)
 
 // Validates W, H, D against the limits every entry point shares.
 int check_dims(int W, int H, int D);
+
+// Winner-takes-all (rts_wta's work) with the caller's per-row scratch for the
+// literal carry (`carry`: H ints; unused, and may be NULL, without
+// RTS_WTA_LITERAL).  Rows at most WTA_MAX_ROWS.
+constexpr int WTA_MAX_ROWS = 65535;
+int wta_launch(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags, int tol,
+               int *carry, hipStream_t s);
+
+// Devices a process can address through per-device tables.
+constexpr int RTS_MAX_DEVICES = 64;
 
 // Cross arms (arm limit at most 17) from a packed 0x00BBGGRR colour image.
 int find_cross_packed(const uint32_t *pcol, uint32_t *arms, int W, int H, int L, int tao, hipStream_t s);

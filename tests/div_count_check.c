@@ -1,6 +1,6 @@
 /* Exhaustive-over-counts check of the count quotient used by the cross
  * sweep: q0 = a*y, r0 = fma(-q0, c, a), q1 = fma(r0, y, q0) with y within two
- * ulp of 1/c, falling back to a/c when q1 is NaN, infinite or denormal, must
+ * ulp of 1/c, falling back to a/c when q1 is NaN, infinite, denormal or zero, must
  * equal the IEEE quotient a/c bit for bit.  The hardware reciprocal is
  * replaced by every float within two ulp of RN(1/c), so the check covers any
  * reciprocal of that accuracy.
@@ -29,7 +29,7 @@ static float quotient(float a, float c, float y) {
     const float r0 = fmaf(-q0, c, a);
     const float q1 = fmaf(r0, y, q0);
     const int cls = fpclassify(q1);
-    if (cls == FP_NAN || cls == FP_INFINITE || cls == FP_SUBNORMAL) return a / c;
+    if (cls == FP_NAN || cls == FP_INFINITE || cls == FP_SUBNORMAL || cls == FP_ZERO) return a / c;
     return q1;
 }
 
@@ -56,7 +56,7 @@ int main(int argc, char **argv) {
     const long samples = argc > 2 ? atol(argv[2]) : 20000;
     rng_state = argc > 3 ? (uint64_t)atoll(argv[3]) * 2654435761u + 1 : 88172645463325252ull;
     long bad = 0, n = 0;
-    const float specials[] = {0.0f, 1.0f, -1.0f, 0x1p-126f, 0x1.fffffep-127f, 0x1p-149f, 0x1p-140f, 0x1p-115f,
+    const float specials[] = {0.0f, -0.0f, 1.0f, -1.0f, 0x1p-126f, 0x1.fffffep-127f, 0x1p-149f, 0x1p-140f, 0x1p-115f,
                               0x1.fffffep127f, -0x1.fffffep127f, INFINITY, -INFINITY, NAN, 3.0f, 1e-30f, 12345.678f};
     for (uint32_t c = 1; c <= cmax; c++) {
         for (size_t i = 0; i < sizeof specials / sizeof specials[0]; i++, n++) bad += check(specials[i], c);

@@ -1,6 +1,6 @@
 """The cross sweep divides each support sum by its integer count with a
 reciprocal, one fused residual correction and an IEEE fallback for
-NaN/infinite/denormal quotients (rtstereo_amd/csrc/k_aggr.hip, div_count).
+NaN/infinite/denormal/zero quotients (rtstereo_amd/csrc/rts_internal.h, div_count).
 This runs the same arithmetic on the CPU for every count the short-arm sweep
 can produce (1..1225) and a sample of counts up to 2^19, with every reciprocal
 within two ulp of 1/c, and requires the IEEE quotient bit for bit

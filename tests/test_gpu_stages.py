@@ -359,6 +359,49 @@ def test_refine(W, H, D, seed):
     assert np.array_equal(S.disp_refine(disp, arms, D, literal=True), oracle.disp_refine(disp, horz, vert, D, literal=True))
 
 
+@pytest.mark.parametrize("W,H,D,seed", [(384, 288, 16, 12), (640, 480, 64, 13)])
+def test_refine_literal_full_size(W, H, D, seed):
+    """The in-place raster-order refinement (anti-diagonal wavefront) at the
+    384x288 workload's size and at 640x480, on a realistic map (the WTA of
+    the colour AD volume) and on a noisy one, bit-exact vs the oracle."""
+    p = _pair(W, H, D, seed)
+    arms, horz, vert = S.find_cross(p.left)
+    disp = oracle.wta(oracle.ad_color(p.left, p.right, D, 1, 20.0))
+    rng = np.random.default_rng(seed)
+    noisy = np.where(rng.random((H, W)) < 0.3, rng.integers(0, D, size=(H, W)), disp).astype(np.uint8)
+    for dmap in (disp, noisy):
+        assert np.array_equal(S.disp_refine(dmap, arms, D, literal=True),
+                              oracle.disp_refine(dmap, horz, vert, D, literal=True))
+        assert np.array_equal(S.disp_refine(dmap, arms, D), oracle.disp_refine(dmap, horz, vert, D))
+
+
+def test_refine_long_arms():
+    """Arms beyond 17 (up to 150): the wavefront's diagonal step grows with the
+    largest horizontal arm, rows wider than 64 columns take extra passes, and
+    one pixel's support of 300 x 301 pixels (more than a 16-bit bin holds)
+    takes the out-of-place kernel's 32-bit recount."""
+    W, H, D = 320, 310, 32
+    print(f"[case] refine long arms {W}x{H} D={D}", flush=True)
+    rng = np.random.default_rng(14)
+    horz = rng.integers(0, 4, size=(H, W, 2)).astype(np.int32)
+    vert = rng.integers(0, 4, size=(H, W, 2)).astype(np.int32)
+    horz[:, 100, :] = 40            # rows 81 columns wide around column 100
+    vert[50:60, 100, :] = 45
+    horz[:, 160, :] = 150           # rows 301 columns wide around column 160
+    vert[155, 160, :] = 150
+    xs, ys = np.arange(W)[None, :], np.arange(H)[:, None]
+    horz[..., 0] = np.minimum(horz[..., 0], xs)            # arms stay inside the image
+    horz[..., 1] = np.minimum(horz[..., 1], W - 1 - xs)
+    vert[..., 0] = np.minimum(vert[..., 0], ys)
+    vert[..., 1] = np.minimum(vert[..., 1], H - ys)
+    arms = (horz[..., 0] | horz[..., 1] << 8 | vert[..., 0] << 16 | vert[..., 1] << 24).astype(np.uint32)
+    disp = rng.integers(0, D, size=(H, W)).astype(np.uint8)
+    disp[:, 150:170] = rng.integers(0, 3, size=(H, 20))    # a clear mode inside the wide support
+    assert np.array_equal(S.disp_refine(disp, arms, D), oracle.disp_refine(disp, horz, vert, D))
+    assert np.array_equal(S.disp_refine(disp, arms, D, literal=True),
+                          oracle.disp_refine(disp, horz, vert, D, literal=True))
+
+
 def test_layout_roundtrip_and_gray():
     rng = np.random.default_rng(1)
     v = rng.random((7, 13, 29)).astype(np.float32)
