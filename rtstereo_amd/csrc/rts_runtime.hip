@@ -55,10 +55,26 @@ __global__ void k_fill_u8(uint8_t *dst, uint8_t v, size_t n) {
     for (; i < n; i += stride) dst[i] = v;
 }
 
-__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) dst[i] = src[i];
+// Copy probe: a block moves rounds of 256 x CP_U 16-byte words, each thread
+// issuing all CP_U loads of a round before its first store (CP_U loads in
+// flight per lane); loads and stores are non-temporal, as a stream's are.
+constexpr int CP_U = 8;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy16(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, size_t n) {
+    constexpr size_t per = 256 * CP_U;
+    for (size_t b = (size_t)blockIdx.x * per; b < n; b += (size_t)gridDim.x * per) {
+        u32x4 v[CP_U];
+#pragma unroll
+        for (int k = 0; k < CP_U; k++) {
+            const size_t i = b + (size_t)k * 256 + threadIdx.x;
+            if (i < n) v[k] = __builtin_nontemporal_load(src + i);
+        }
+#pragma unroll
+        for (int k = 0; k < CP_U; k++) {
+            const size_t i = b + (size_t)k * 256 + threadIdx.x;
+            if (i < n) __builtin_nontemporal_store(v[k], dst + i);
+        }
+    }
 }
 
 __global__ void k_rgb_to_gray(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ gray, int n) {
@@ -245,8 +261,8 @@ rts_status rts_vol_to_pm(const float *dm, float *pm, int W, int H, int D, void *
 rts_status rts_copy_kernel(const void *src, void *dst, size_t bytes, void *stream) {
     RTS_CHECK(bytes % 16 == 0, "copy size %zu is not a multiple of 16", bytes);
     const size_t n = bytes / 16;
-    const int blocks = (int)std::min<size_t>((n + 255) / 256, 256 * 64);
-    k_copy16<<<blocks, 256, 0, S(stream)>>>((const uint4 *)src, (uint4 *)dst, n);
+    const int blocks = (int)std::min<size_t>((n + 256 * CP_U - 1) / (256 * CP_U), 256 * 16);
+    k_copy16<<<blocks, 256, 0, S(stream)>>>((const u32x4 *)src, (u32x4 *)dst, n);
     RTS_LAUNCHED();
     return RTS_OK;
 }
