@@ -260,6 +260,17 @@ enum { RTS_WTA_MAX = 1, RTS_WTA_LITERAL = 2 };
 rts_status rts_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags,
                    int tol, void *stream);
 
+/* rts_wta that allocates nothing: the literal carry's per-row scratch (the
+ * only device memory rts_wta uses beyond its arguments) is a fixed array
+ * compiled into the library, one per device, of 65535 rows; H above that is
+ * rejected with RTS_ERR_INVALID.  Results equal rts_wta's.  Calls on one
+ * device share that array, so calls that may overlap on one device (other
+ * streams, other threads) race: order them.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops). */
+rts_status rts_wta_noalloc(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags,
+                           int tol, void *stream);
+
 /* Left-right check of two disparity maps (tol 0 = the reference's equality).
  * seq/CrossCheck.cpp:10-23 cross_check. */
 rts_status rts_cross_check(const uint8_t *dl, const uint8_t *dr, uint8_t *mask, int W, int H, int tol, void *stream);

@@ -85,6 +85,7 @@ _SIGS = {
     "rts_cross_v": [vp, vp, vp, vp, i32, i32, i32, i32, vp],
     "rts_cross_fused": [vp, vp, vp, vp, i32, i32, i32, vp],
     "rts_wta": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],
+    "rts_wta_noalloc": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],
     "rts_cross_check": [vp, vp, vp, i32, i32, i32, vp],
     "rts_disp_refine": [vp, vp, vp, i32, i32, i32, i32, vp],
     "rts_pipeline_defaults": [C.POINTER(PipelineDesc), i32, i32, i32],

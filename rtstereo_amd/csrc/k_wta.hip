@@ -3,6 +3,8 @@
 // refinement.
 #include "rts_internal.h"
 
+#include <mutex>
+
 namespace rts {
 
 // A block of T threads owns a chunk of T*PPT pixels of one row; thread t
@@ -409,6 +411,34 @@ int wta_launch(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W,
     return RTS_OK;
 }
 
+// Scratch of rts_wta_noalloc: the per-row carries of the literal WTA.  Each
+// device holds its own copy for the life of the process; host code reaches it
+// only through the address hipGetSymbolAddress returns for the current
+// device (the array's name is not a device address on the host).
+static __device__ __align__(256) int scratch[WTA_MAX_ROWS];
+
+namespace {
+std::mutex g_scratch_mu;
+int *g_scratch_ptr[RTS_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up on the
+// device's first call.
+int scratch_ptr(int **out) {
+    int dev = 0;
+    RTS_HIP(hipGetDevice(&dev));
+    RTS_CHECK(dev >= 0 && dev < RTS_MAX_DEVICES, "device %d beyond the %d supported", dev, RTS_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        RTS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RTS_CHECK(p != nullptr, "no device address for the WTA scratch");
+        g_scratch_ptr[dev] = (int *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return RTS_OK;
+}
+}  // namespace
+
 }  // namespace rts
 
 using namespace rts;
@@ -424,6 +454,15 @@ rts_status rts_wta(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, in
     const int st = wta_launch(vol, dl, dr, mask, W, H, D, flags, tol, carry, s);
     if (carry) (void)hipFreeAsync(carry, s);
     return st;
+}
+
+rts_status rts_wta_noalloc(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags,
+                           int tol, void *stream) {
+    RTS_CHECK(H >= 1 && H <= WTA_MAX_ROWS, "rts_wta_noalloc: height %d outside 1..%d (its scratch holds %d rows)", H,
+              WTA_MAX_ROWS, WTA_MAX_ROWS);
+    int *carry = nullptr;
+    if (int e = scratch_ptr(&carry)) return e;
+    return wta_launch(vol, dl, dr, mask, W, H, D, flags, tol, carry, S(stream));
 }
 
 rts_status rts_cross_check(const uint8_t *dl, const uint8_t *dr, uint8_t *mask, int W, int H, int tol, void *stream) {
