@@ -200,6 +200,7 @@ def main() -> int:
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--no-aw", action="store_true", help="skip the adaptive-weight FP32/FP16 timing")
     ap.add_argument("--no-configs", action="store_true", help="skip the 384x288 plain and 720p 4-pass timings")
+    ap.add_argument("--aw-only", action="store_true", help="print only the adaptive-weight FP32/FP16 timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has one GPU, "
@@ -213,6 +214,9 @@ def main() -> int:
     if gpu_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.aw_only:
+        print(json.dumps({"aw": aw_timing(Stream())}))
+        return 0
     pair = gen.census_cross_1080p()
     stream = Stream()
     left, right = DeviceArray.from_host(pair.left), DeviceArray.from_host(pair.right)

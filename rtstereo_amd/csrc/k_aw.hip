@@ -45,7 +45,10 @@ __host__ inline int aw_rmax_for(int r) { return r <= 3 ? 3 : r <= 5 ? 5 : r <= 8
 // Weights of band rows [y0, y0+rows) of one view: wb[ry][dy+r][x][PG], the
 // NT words of a pixel's window row contiguous and padded to PG (a multiple
 // of 4) so that a row is whole 16-byte words.  Block: 64 pixels x 4 window
-// rows; each thread computes all taps of one (pixel, window row).
+// rows; each thread computes all taps of one (pixel, window row).  The
+// window row's Lab pixels (columns x0-R-1 .. x0+64+R) are staged in LDS as
+// packed words, and the block's output rows (each 64*PG contiguous words)
+// are written back through LDS as whole 16-byte words by consecutive lanes.
 // prox(dy, dx) = exp(-sqrt(dy^2+dx^2)/gp) comes from a per-block table; the
 // colour term is exp2(|dLab| * kc) with kc = -log2(e)/gc (hardware exp2 and
 // square root, each within an ulp or two: relative weight error ~1e-6).
@@ -54,23 +57,36 @@ __global__ void __launch_bounds__(256) k_aw_weights_band(const uint8_t *__restri
                                                          int W, int H, int Wp, int y0, int nrow, int r, float kc,
                                                          float gp) {
     constexpr int NT = HALF ? R + 1 : 2 * R + 1, PG = (NT + 3) & ~3;
+    constexpr int SW = 64 + 2 * R + 2;   // staged columns x0-R-1 .. x0+64+R
+    constexpr uint32_t NONE = 0xFF000000u;   // staged column outside the image
     __shared__ float prox[4][2 * R + 2];
+    __shared__ uint32_t pix[4][SW];
+    __shared__ __attribute__((aligned(16))) uint32_t outw[4][64 * PG];
     const int e = 2 * r + 1;
-    const int row = blockIdx.y * 4 + (threadIdx.x >> 6);   // ry * e + (dy + r)
+    const int g = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.y * 4 + g;   // ry * e + (dy + r)
+    const bool rowok = row < nrow;
     const int ry = row / e, dy = row - ry * e - r;
     const int y = y0 + ry, yy = y + dy;
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    if ((threadIdx.x & 63) < 2 * R + 2) {
-        const int dx = (int)(threadIdx.x & 63) - R;
-        prox[threadIdx.x >> 6][threadIdx.x & 63] =
-            (dx > r || dx < -r || dy > r || dy < -r) ? 0.0f : expf(-sqrtf((float)(dy * dy + dx * dx)) / gp);
+    const int x0 = blockIdx.x * 64, x = x0 + lane;
+    if (lane < 2 * R + 2) {
+        const int dx = lane - R;
+        prox[g][lane] = (dx > r || dx < -r || dy > r || dy < -r) ? 0.0f : expf(-sqrtf((float)(dy * dy + dx * dx)) / gp);
+    }
+    const bool yok = rowok && yy >= 0 && yy < H;
+    for (int i = lane; i < SW; i += 64) {
+        const int xx = x0 - R - 1 + i;
+        uint32_t v = NONE;
+        if (yok && xx >= 0 && xx < W) {
+            const uint8_t *t = lab + 3 * ((size_t)yy * W + xx);
+            v = (uint32_t)t[0] | (uint32_t)t[1] << 8 | (uint32_t)t[2] << 16;
+        }
+        pix[g][i] = v;
     }
     __syncthreads();
-    if (row >= nrow) return;
-    const float *pr = prox[threadIdx.x >> 6];
-    const bool ok = x < W && yy >= 0 && yy < H;
+    const float *pr = prox[g];
+    const bool ok = yok && x < W;
     float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-    const uint8_t *nrowp = lab + 3 * (size_t)(ok ? yy : 0) * W;
     if (ok) {
         const uint8_t *c = lab + 3 * ((size_t)y * W + x);
         c0 = c[0];
@@ -78,20 +94,20 @@ __global__ void __launch_bounds__(256) k_aw_weights_band(const uint8_t *__restri
         c2 = c[2];
     }
     auto weight = [&](int dx) {
-        const int xx = x + dx;
-        if (!ok || dx > R || xx < 0 || xx >= W) return 0.0f;
-        const uint8_t *t = nrowp + 3 * xx;
-        const float e0 = c0 - (float)t[0], e1 = c1 - (float)t[1], e2 = c2 - (float)t[2];
+        if (!ok || dx > R) return 0.0f;
+        const uint32_t t = pix[g][lane + R + 1 + dx];
+        if (t == NONE) return 0.0f;
+        const float e0 = c0 - (float)(t & 255u), e1 = c1 - (float)((t >> 8) & 255u), e2 = c2 - (float)(t >> 16);
         const float dist = __builtin_amdgcn_sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
         return pr[dx + R] * __builtin_amdgcn_exp2f(dist * kc);
     };
-    uint4 *dst = reinterpret_cast<uint4 *>(wb + ((size_t)row * Wp + x) * PG);
+    uint4 *mine = reinterpret_cast<uint4 *>(&outw[g][lane * PG]);
 #pragma unroll
-    for (int g = 0; g < PG / 4; g++) {
+    for (int q = 0; q < PG / 4; q++) {
         uint32_t w4[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int jt = 4 * g + i;
+            const int jt = 4 * q + i;
             if (jt >= NT) {
                 w4[i] = 0u;
             } else if (HALF) {
@@ -103,7 +119,14 @@ __global__ void __launch_bounds__(256) k_aw_weights_band(const uint8_t *__restri
                 w4[i] = __float_as_uint(weight(jt - R));
             }
         }
-        dst[g] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        mine[q] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+    }
+    __syncthreads();
+    // row g of the block: 64*PG contiguous words at (row, x0)
+    if (rowok) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(outw[g]);
+        uint4 *dst = reinterpret_cast<uint4 *>(wb + ((size_t)row * Wp + x0) * PG);
+        for (int i = lane; i < 16 * PG; i += 64) dst[i] = src[i];
     }
 }
 
