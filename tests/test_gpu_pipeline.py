@@ -144,6 +144,10 @@ def test_census_cross_1080p_properties(exact):
     ("plain", 96, 64, 16, {}),
     ("aw", 80, 48, 16, {"aw_r": 5}),
     ("rank_log_ncc", 96, 64, 24, {}),
+    # D >= 128: the captured frame runs in disparity chunks on three streams
+    ("census_cross", 320, 96, 192, {}),
+    ("census_cross", 260, 80, 130, {"wta_literal": 1}),
+    ("census_cross", 256, 72, 256, {"lr_tol": 0}),
 ])
 def test_pipeline_graph_replay(name, W, H, D, over):
     """Graph replay: runs with the same buffers and stream are captured once
