@@ -465,6 +465,30 @@ rts_status rts_rgb_to_lab(const uint8_t *rgb, uint8_t *lab, int W, int H, void *
  * (opencl/Main.cpp:342-345). */
 rts_status rts_median3(const uint8_t *rgb, uint8_t *out, int W, int H, void *stream);
 
+/* ------------------------------------------------------------ image files */
+
+/* PNG input in the layout of the reference driver's OpenCV imread
+ * (seq/Main.cpp:72-73 with flag 0, :87-88 with flag 1): channels = 1 gives
+ * gray [H][W], channels = 3 gives B,G,R [H][W][3].  Every PNG colour type and
+ * bit depth is read: 16-bit samples keep their high byte, 1/2/4-bit gray
+ * levels are scaled to 0..255, palettes are expanded, alpha is dropped,
+ * Adam7 interlacing is undone; colour files read as gray take
+ * (29B + 150G + 77R + 128) >> 8 (rts_rgb_to_gray's rule).  *pixels is host
+ * memory owned by the caller, released with rts_png_free.  Chunk CRCs and
+ * the image data's length are checked (RTS_ERR_INVALID on a damaged file). */
+rts_status rts_png_read(const char *path, int channels, uint8_t **pixels, int *W, int *H);
+/* The same from a PNG file already in memory. */
+rts_status rts_png_decode(const uint8_t *data, size_t bytes, int channels, uint8_t **pixels, int *W, int *H);
+void rts_png_free(uint8_t *pixels);
+/* 8-bit PNG output of a gray [H][W] (channels = 1) or B,G,R [H][W][3]
+ * (channels = 3) host image: the reference's imwrite of its disparity image
+ * (seq/Main.cpp:275-280). */
+rts_status rts_png_write(const char *path, const uint8_t *pixels, int W, int H, int channels);
+/* Display scaling of a host disparity map before writing, in place:
+ * v <- min(round(v * coeff), 255), with coeff <= 0 meaning 255 / max(v)
+ * (a map of zeros is left as it is).  seq/Main.cpp:335-360 scalecvMat. */
+rts_status rts_disp_scale(uint8_t *disp, int W, int H, float coeff);
+
 #ifdef __cplusplus
 }
 #endif

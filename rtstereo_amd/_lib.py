@@ -105,8 +105,13 @@ _SIGS = {
     "rts_rgb_to_gray": [vp, vp, i32, i32, vp],
     "rts_rgb_to_lab": [vp, vp, i32, i32, vp],
     "rts_median3": [vp, vp, i32, i32, vp],
+    "rts_png_read": [cp, i32, C.POINTER(vp), C.POINTER(i32), C.POINTER(i32)],
+    "rts_png_decode": [vp, sz, i32, C.POINTER(vp), C.POINTER(i32), C.POINTER(i32)],
+    "rts_png_free": [vp],
+    "rts_png_write": [cp, vp, i32, i32, i32],
+    "rts_disp_scale": [vp, i32, i32, f32],
 }
-_RESTYPES = {"rts_last_error": cp, "rts_version": cp, "rts_pipeline_defaults": None,
+_RESTYPES = {"rts_last_error": cp, "rts_version": cp, "rts_pipeline_defaults": None, "rts_png_free": None,
              "rts_pipeline_bytes": sz, "rts_pipeline_volume": vp,
              "rts_aw_workspace_bytes": sz, "rts_cost_rank_log_ncc_ws_bytes": sz}
 

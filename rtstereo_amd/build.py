@@ -19,6 +19,7 @@ ARCH = os.environ.get("RTS_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["rts_runtime.hip", "k_cost.hip", "k_aggr.hip", "k_aw.hip", "k_wta.hip", "pipeline.hip",
                "server.hip", "batch.hip"]
+HOST_SOURCES = ["png_io.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
              # packed FP32 instructions are kept out of every kernel
@@ -47,8 +48,13 @@ def build(verbose: bool = False) -> list[str]:
         for out in ex.map(compile_one, zip(srcs, objs)):
             if verbose and out.strip():
                 print(out)
+    # host-only sources (image files on zlib)
+    for src in HOST_SOURCES:
+        obj = os.path.join(OBJ, src.replace(".cpp", ".o"))
+        _run(["g++", "-O2", "-fPIC", "-std=c++17", "-Wall", f"-I{INCLUDE}", "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     lib = os.path.join(HERE, "librtstereo.so")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib + ".tmp", *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", lib + ".tmp", *objs, "-lz"])
     os.replace(lib + ".tmp", lib)
     seq = os.path.join(HERE, "librtstereo_seq.so")
     _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", f"-I{INCLUDE}",
