@@ -47,11 +47,28 @@ STAGE_BYTES_PER_PD = {"cost": 4.0, "aggr": 8.0, "wta": 4.0}
 # read + write 8, V pass read + write 8, WTA read 4.
 FRAME_CONTRACT_BYTES_PER_PD = 24.0
 STAGE_KERNEL = {"cost": "k_front_hscan", "aggr": "k_cross_sweep", "wta": "k_wta"}
-# HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by
-# the copy kernel's reading of the same counters (FETCH x2.000, WRITE x1.000);
-# profiles/census_cross_1080p_d192.md holds the runs.
-STAGE_TRAFFIC = {"cost": (24424.9 * 2 + 1555212.8) * 1024, "aggr": (1215821.1 * 2 + 1555200.1) * 1024,
-                 "wta": (805732.9 * 2 + 4050.0) * 1024}
+# The frame as built moves 16 B/pd: prefix volume written (4) and read (4),
+# aggregated volume written (4) and read (4).
+FRAME_BYTES_PER_PD = 16.0
+# HBM bytes per dispatch from rocprofv3 FETCH_SIZE / WRITE_SIZE (KiB), scaled
+# by the copy kernel's reading of the same counters (FETCH x2.000, WRITE
+# x1.000).  Not measured in this process (counters need the profiler): the
+# figures of the profile run recorded in profiles/census_cross_1080p_d192.md,
+# re-taken whenever one of these kernels changes.
+STAGE_TRAFFIC = {"cost": (24430 * 2 + 1555213) * 1024, "aggr": (1212000 * 2 + 1555200) * 1024,
+                 "wta": (804500 * 2 + 4050) * 1024}
+TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/census_cross_1080p_d192.md"
+# Adaptive weights: 1089 taps per pd; 4 flops per tap (weight product, the
+# denominator's add, the numerator's multiply-add); the VALU issue count per
+# tap without packed FP32 is 3 (FP32: product, add, fused multiply-add) or
+# 1.5 (FP16: one packed product and two dot2 per tap pair); one VALU
+# instruction is 64 lanes, 256 CUs x 4 SIMDs issue one every 4 cycles at
+# 2.4 GHz.
+AW_TAPS = 33 * 33
+AW_FLOP_PER_TAP = 4.0
+AW_VALU_PER_TAP = {"fp32": 3.0, "fp16": 1.5}
+PEAK_VECTOR_TFLOPS = 157.3
+VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9
 
 
 def cpu_model() -> str:
@@ -128,10 +145,12 @@ def cpu_baseline(target_s: float) -> dict:
     return single, parallel
 
 
-def aw_timing(stream) -> dict:
+def aw_timing(stream, rows=None) -> dict:
     """Adaptive-weight pipeline (truncated AD 40, 33x33 window, gc 7, gp 36,
     WTA + check) at 1920x1080, D=192, FP32 and FP16: mean of 3 frames after
-    one warm-up, Lab derived on the device."""
+    one warm-up, Lab derived on the device.  With `rows`, each precision's
+    aggregated volume on those rows is kept (key "rows") for the accuracy
+    check."""
     from rtstereo_amd import gen, pipeline as PL, _lib as L
     from rtstereo_amd.device import DeviceArray, Event
     p = gen.aw_1080p()
@@ -150,11 +169,51 @@ def aw_timing(stream) -> dict:
         e1.record(stream.handle)
         stream.sync()
         frame = e0.elapsed_ms(e1) / 3
-        res[name] = {"frame_ms": frame, "aggregation_ms": float(np.mean(agg)), "pd_per_s": W * H * D / (frame / 1e3)}
+        aggr = float(np.mean(agg))
+        taps = W * H * D * AW_TAPS
+        flops = taps * AW_FLOP_PER_TAP
+        valu_ms = taps * AW_VALU_PER_TAP[name] / VALU_LANE_OPS_PER_S * 1e3
+        res[name] = {"frame_ms": frame, "aggregation_ms": aggr, "pd_per_s": W * H * D / (frame / 1e3),
+                     "roofline": {"bound": "valu", "flop_per_tap": AW_FLOP_PER_TAP, "taps_per_pd": AW_TAPS,
+                                  "algorithmic_flops": flops, "achieved": flops / (aggr / 1e3) / 1e12,
+                                  "peak": PEAK_VECTOR_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": flops / (aggr / 1e3) / 1e12 / PEAK_VECTOR_TFLOPS,
+                                  "valu_instr_per_tap": AW_VALU_PER_TAP[name], "valu_bound_ms": valu_ms,
+                                  "frac_of_valu_bound": valu_ms / aggr}}
+        if rows is not None:
+            vol = pipe.volume()
+            res[name]["rows"] = np.moveaxis(vol[:, rows, :], 0, -1).astype(np.float64)
+            del vol
         pipe.close()
     a.free()
     b.free()
     return res
+
+
+AW_ROWS = sorted(set([0, 1, 57, 58, 103, 104, 115, 116, 207, 208, 1078, 1079] +
+                     list(np.linspace(2, 1077, 40).astype(int))))
+
+
+def aw_accuracy(aw: dict) -> dict:
+    """The AW volumes of AW_ROWS (52 whole rows, 99,840 pixels x 192
+    disparities) against the FP64 oracle, as the checker (CPU; part of the
+    CPU-baseline leg): FP32 max relative error, FP16 max / mean absolute
+    error, and the fraction of pixels whose winning disparity differs."""
+    import oracle
+    from rtstereo_amd import gen
+    p = gen.aw_1080p()
+    ll, lr = p.lab()
+    cost = oracle.ad_color(p.left, p.right, D, 1, 40.0, False)
+    f64 = oracle.aw_rows_f64(ll, lr, cost, np.array(AW_ROWS), 16, 7.0, 36.0)
+    d64 = np.argmin(f64, -1)
+    out = {"sample": f"{len(AW_ROWS)} whole rows ({len(AW_ROWS) * W} pixels x {D} disparities) against the FP64 oracle"}
+    for name in ("fp32", "fp16"):
+        got = aw[name].pop("rows")
+        err = np.abs(got - f64)
+        out[name] = {"max_abs_err": float(err.max()), "mean_abs_err": float(err.mean()),
+                     "max_rel_err": float(np.max(err / np.abs(f64))),
+                     "disparity_disagreement": float((np.argmin(got, -1) != d64).mean())}
+    return out
 
 
 def config_timing(stream, frames: int = 50) -> dict:
@@ -303,7 +362,13 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": STAGE_KERNEL[dom], "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": STAGE_TRAFFIC[dom],
                      "bytes_per_pd": STAGE_BYTES_PER_PD[dom], "algorithmic_bytes_per_launch": bytes_launch,
-                     "kernel_ms": stage[dom]},
+                     "kernel_ms": stage[dom], "traffic_source": TRAFFIC_SOURCE},
+        # The whole frame on the bytes it moves (16 B/pd: prefix volume
+        # written and read, aggregated volume written and read).
+        "roofline_frame": {"bytes_per_pd": FRAME_BYTES_PER_PD,
+                           "achieved": FRAME_BYTES_PER_PD * pd / (total_ms / args.steps / 1e3) / 1e9,
+                           "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": FRAME_BYTES_PER_PD * pd / (total_ms / args.steps / 1e3) / 1e9 / PEAK_HBM_GBS},
         # The whole frame against the reference dataflow's compulsory HBM
         # bytes (cost volume write, H and V pass read + write, WTA read:
         # 24 B/pd at one iteration).  The fused front end never moves the
@@ -316,7 +381,7 @@ def main() -> int:
         "copy_kernel_gbs": copy,
     }
     if not args.no_aw:
-        out["aw"] = aw_timing(stream)
+        out["aw"] = aw_timing(stream, AW_ROWS if args.cpu_seconds > 0 else None)
     if not args.no_configs:
         out["other_configs"] = config_timing(stream)
     if args.cpu_seconds > 0:
@@ -324,6 +389,8 @@ def main() -> int:
         out["cpu_baseline"] = cb
         out["cpu_baseline_all_cores"] = cbn
         out["speedup_vs_cpu"] = value / cb["value"]
+        if "aw" in out:
+            out["aw"]["accuracy"] = aw_accuracy(out["aw"])
     print(json.dumps(out))
     return 0
 

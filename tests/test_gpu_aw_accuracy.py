@@ -1,6 +1,6 @@
 """Adaptive-weight aggregation at full size (1920x1080, D=192, 33x33 window,
-truncated AD 40, gamma_c 7, gamma_p 36) against the FP64 oracle on 52 whole
-rows (99,840 pixels x 192 disparities): the FP32 form within 1e-5 relative,
+truncated AD 40, gamma_c 7, gamma_p 36) against the FP64 oracle on 51 whole
+rows (97,920 pixels x 192 disparities): the FP32 form within 1e-5 relative,
 the FP16 form's largest cost error and disparity disagreement reported and
 bounded."""
 import numpy as np
