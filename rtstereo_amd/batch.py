@@ -1,7 +1,8 @@
 """Batched pairs of mixed size over the C ABI (rts_batch_*): the rank + LoG +
 NCC cost with box aggregation, chunked under a device budget, the box stage
-as one persistent work-queue launch per chunk, each chunk captured into a
-HIP graph and replayed."""
+as one persistent work-queue launch per chunk; any other configuration
+(adaptive weights, census + cross) through resident per-shape pipelines;
+each chunk captured into a HIP graph and replayed."""
 from __future__ import annotations
 
 import ctypes as C
@@ -39,7 +40,14 @@ for _n, (_a, _r) in _SIGS.items():
 
 
 def pair_bytes(W: int, H: int, D: int) -> int:
+    """Device bytes of one pair in the rank + LoG + NCC batch path."""
     return int(lib.rts_batch_pair_bytes(W, H, D))
+
+
+def pipeline_bytes(desc: PipelineDesc) -> int:
+    """Device bytes of one resident pipeline of the batch's pipeline path
+    (the other configurations): one per pair shape."""
+    return int(lib.rts_pipeline_bytes(C.byref(desc)))
 
 
 class Pair:
@@ -90,12 +98,22 @@ class Batch:
             pass
 
 
+CONFIGS = {
+    "rank_log_ncc": ("rank_log_ncc", {}),
+    "aw_fp16": ("aw", {"aw_precision": L.RTS_AW_FP16}),
+    "aw_fp32": ("aw", {"aw_precision": L.RTS_AW_FP32}),
+    "census_cross": ("census_cross", {}),
+}
+
+
 def main(argv=None) -> int:
     """Batch measurement: n seeded pairs of the uneven workload (sizes from
     640x480 to 2560x1440, d_max 64..256), inputs resident on the device.
-    One run builds the graphs and tunes; the timed runs replay them.  The
-    same pairs then run one at a time through resident per-shape pipelines
-    for comparison.  Prints one JSON object."""
+    One run builds the graphs (and, for rank_log_ncc, tunes); the timed runs
+    replay them.  The same pairs then run one at a time through resident
+    per-shape pipelines of the baseline configuration (by default the same
+    one; e.g. --config aw_fp16 --baseline aw_fp32 compares the batched FP16
+    form with per-frame FP32).  Prints one JSON object."""
     import argparse
     import json
     import time
@@ -103,13 +121,17 @@ def main(argv=None) -> int:
     from . import pipeline as PL
     from .device import Event, Stream
     ap = argparse.ArgumentParser()
-    ap.add_argument("--pairs", type=int, default=256)
+    ap.add_argument("--config", choices=sorted(CONFIGS), default="rank_log_ncc")
+    ap.add_argument("--baseline", choices=sorted(CONFIGS), default=None)
+    ap.add_argument("--pairs", type=int, default=None, help="default 256 (32 for adaptive weights)")
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--runs", type=int, default=3)
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args(argv)
-    shapes = gen.batch_shapes(args.pairs, args.seed)
+    npairs = args.pairs or (32 if args.config.startswith("aw") else 256)
+    base_name = args.baseline or args.config
+    shapes = gen.batch_shapes(npairs, args.seed)
     t0 = time.perf_counter()
     made = {}
     pairs = []
@@ -121,7 +143,8 @@ def main(argv=None) -> int:
     gen_s = time.perf_counter() - t0
     pd = sum(W * H * D for W, H, D in shapes)
     s = Stream()
-    b = Batch(PL.preset("rank_log_ncc", 64, 64, 16), int(args.budget_gb * 2**30), not args.no_graphs)
+    name, over = CONFIGS[args.config]
+    b = Batch(PL.preset(name, 64, 64, 16, **over), int(args.budget_gb * 2**30), not args.no_graphs)
     t0 = time.perf_counter()
     b.run(pairs, s.handle)
     s.sync()
@@ -137,10 +160,14 @@ def main(argv=None) -> int:
     st = b.stats()
     batched = [p.results() for p in pairs]
     b.close()
+    bname, bover = CONFIGS[base_name]
     pipes = {}
     for W, H, D in shapes:
         if (W, H, D) not in pipes:
-            pipes[(W, H, D)] = PL.Pipeline(PL.preset("rank_log_ncc", W, H, D))
+            pipes[(W, H, D)] = PL.Pipeline(PL.preset(bname, W, H, D, **bover))
+    for p, key in zip(pairs[:1], shapes[:1]):   # warm-up
+        pipes[key].run(p.left, p.right, stream=s.handle)
+    s.sync()
     e0.record(s.handle)
     for p, key in zip(pairs, shapes):
         pipes[key].run(p.left, p.right, stream=s.handle)
@@ -152,15 +179,17 @@ def main(argv=None) -> int:
         agree.append(float((pipes[key].results()[0] == dl).mean()))
     for q in pipes.values():
         q.close()
-    out = {"pairs": args.pairs, "distinct_shapes": len(made), "pd": pd, "batch_ms": ms,
-           "batch_pd_per_s": pd / (min(ms) / 1e3), "pairs_per_s": args.pairs / (min(ms) / 1e3),
-           "single_pipelines_ms": single_ms, "single_pd_per_s": pd / (single_ms / 1e3),
+    out = {"config": args.config, "baseline": base_name, "pairs": npairs, "distinct_shapes": len(made), "pd": pd,
+           "batch_ms": ms, "batch_pd_per_s": pd / (min(ms) / 1e3), "pairs_per_s": npairs / (min(ms) / 1e3),
+           "mean_ms_per_pair": min(ms) / npairs,
+           "baseline_single_pipelines_ms": single_ms, "baseline_pd_per_s": pd / (single_ms / 1e3),
            "speedup": single_ms / min(ms), "first_run_s": first_s,
-           "disparity_agreement_batched_vs_single": {"mean": float(np.mean(agree)), "min": float(np.min(agree))},
-           "chunks": st.chunks, "arena_bytes": st.arena_bytes, "graphs_built": st.graphs_built,
-           "graph_replays": st.graph_replays, "tuned": {"box_rows": st.box_rows, "cost_disparities": st.cost_disparities,
-                                                         "tune_ms": [round(x, 3) for x in st.tune_ms]},
-           "input_generation_s": gen_s, "graphs": not args.no_graphs}
+           "disparity_agreement_batched_vs_baseline": {"mean": float(np.mean(agree)), "min": float(np.min(agree))},
+           "chunks": st.chunks, "device_bytes": st.arena_bytes, "graphs_built": st.graphs_built,
+           "graph_replays": st.graph_replays, "input_generation_s": gen_s, "graphs": not args.no_graphs}
+    if args.config == "rank_log_ncc":
+        out["tuned"] = {"box_rows": st.box_rows, "cost_disparities": st.cost_disparities,
+                        "tune_ms": [round(x, 3) for x in st.tune_ms]}
     print(json.dumps(out))
     return 0
 

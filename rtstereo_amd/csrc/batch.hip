@@ -14,6 +14,13 @@
 // The chunk's whole launch sequence is captured into a HIP graph the first
 // time the same list of pairs (sizes and buffers) is run and replayed after.
 //
+// Any other configuration (adaptive weights in FP16 or FP32, census +
+// cross-based aggregation, ...) runs through resident pipelines, one per
+// pair shape (W, H, D): chunks are cut so that the pipelines of a chunk's
+// shapes fit the budget, and a chunk's whole launch sequence (every pair
+// through its shape's pipeline, in order) is captured into one HIP graph
+// and replayed like the specialised path's.
+//
 // The reference has no counterpart: its driver runs one pair per process
 // (seq/Main.cpp:47-299).
 #include "rts_internal.h"
@@ -23,6 +30,8 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <set>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -212,9 +221,22 @@ bool same_items(const std::vector<rts_batch_item> &a, const rts_batch_item *b, c
 
 }  // namespace
 
+// A chunk of the pipeline path: its pairs and the graph of their launches.
+struct PipeChunk {
+    std::vector<int> items;
+    std::vector<rts_batch_item> key;
+    hipGraphExec_t exec = nullptr;
+};
+
+typedef std::tuple<int, int, int> Shape;   // W, H, D
+
 struct rts_batch {
     rts_pipeline_desc base{};
     size_t budget = 0;
+    // the pipeline path: resident pipelines by shape, and its chunks
+    bool generic = false;
+    std::map<Shape, rts_pipeline *> pipes;
+    std::vector<std::unique_ptr<PipeChunk>> pchunks;
     // One arena, sized for the largest chunk, serves every chunk in turn:
     // chunks run one after another on the stream, so their graphs may all
     // point into it.
@@ -429,21 +451,155 @@ int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s)
     return st;
 }
 
+rts_pipeline_desc shape_desc(const rts_batch *b, const Shape &k) {
+    rts_pipeline_desc d = b->base;
+    d.width = std::get<0>(k);
+    d.height = std::get<1>(k);
+    d.disp = std::get<2>(k);
+    return d;
+}
+
+size_t shape_bytes(const rts_batch *b, const Shape &k) {
+    const rts_pipeline_desc d = shape_desc(b, k);
+    return rts_pipeline_bytes(&d);
+}
+
+void drop_pchunk(PipeChunk *c) {
+    if (c->exec) (void)hipGraphExecDestroy(c->exec);
+    c->exec = nullptr;
+}
+
+// The pipeline path of rts_batch_run.
+int run_generic(rts_batch *b, const rts_batch_item *items, int n, hipStream_t s) {
+    // chunks: consecutive pairs while the pipelines of their distinct shapes
+    // fit the budget
+    std::vector<std::vector<int>> plan;
+    std::vector<std::set<Shape>> plan_shapes;
+    size_t used = 0;
+    for (int i = 0; i < n; i++) {
+        const Shape k{items[i].width, items[i].height, items[i].disp};
+        const size_t need = shape_bytes(b, k);
+        RTS_CHECK(need <= b->budget, "pair %d needs %zu bytes, over the budget of %zu", i, need, b->budget);
+        const bool have = !plan.empty() && plan_shapes.back().count(k);
+        if (plan.empty() || (!have && used + need > b->budget)) {
+            plan.emplace_back();
+            plan_shapes.emplace_back();
+            used = 0;
+        }
+        plan.back().push_back(i);
+        if (!have) {
+            plan_shapes.back().insert(k);
+            used += need;
+        }
+    }
+    while (b->pchunks.size() < plan.size()) b->pchunks.emplace_back(new PipeChunk());
+    b->stats.pairs = n;
+    b->stats.chunks = (int)plan.size();
+    for (size_t c = 0; c < plan.size(); c++) {
+        PipeChunk &ch = *b->pchunks[c];
+        // make room: pipelines of other shapes leave while the resident set
+        // with this chunk's new shapes would exceed the budget (every graph
+        // that used them goes with them)
+        size_t resident = 0, add = 0;
+        for (auto &kv : b->pipes) resident += shape_bytes(b, kv.first);
+        for (const Shape &k : plan_shapes[c])
+            if (!b->pipes.count(k)) add += shape_bytes(b, k);
+        if (resident + add > b->budget) {
+            RTS_HIP(hipStreamSynchronize(s));
+            for (auto it = b->pipes.begin(); it != b->pipes.end() && resident + add > b->budget;) {
+                if (plan_shapes[c].count(it->first)) {
+                    ++it;
+                    continue;
+                }
+                resident -= shape_bytes(b, it->first);
+                rts_pipeline_destroy(it->second);
+                it = b->pipes.erase(it);
+                for (auto &pc : b->pchunks) {
+                    drop_pchunk(pc.get());
+                    pc->key.clear();
+                }
+            }
+        }
+        for (const Shape &k : plan_shapes[c])
+            if (!b->pipes.count(k)) {
+                const rts_pipeline_desc d = shape_desc(b, k);
+                rts_pipeline *p = nullptr;
+                if (int e = rts_pipeline_create(&d, &p)) return e;
+                b->pipes[k] = p;
+            }
+        auto enqueue = [&]() -> int {
+            for (int i : plan[c]) {
+                const rts_batch_item &it = items[i];
+                rts_pipeline *p = b->pipes[Shape{it.width, it.height, it.disp}];
+                if (int e = pipeline_enqueue(p, it.left_rgb, it.right_rgb, it.dl, it.dr, it.mask, s)) return e;
+            }
+            return (int)RTS_OK;
+        };
+        if (!b->use_graphs) {
+            if (int e = enqueue()) return e;
+            continue;
+        }
+        if (ch.exec && same_items(ch.key, items, plan[c])) {
+            RTS_HIP(hipGraphLaunch(ch.exec, s));
+            b->stats.graph_replays++;
+            continue;
+        }
+        drop_pchunk(&ch);
+        ch.items = plan[c];
+        ch.key.clear();
+        for (int i : ch.items) ch.key.push_back(items[i]);
+        hipGraph_t g = nullptr;
+        RTS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int e = enqueue();
+        const hipError_t ce = hipStreamEndCapture(s, &g);
+        if (e != RTS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            ch.key.clear();
+            return e;
+        }
+        RTS_HIP(ce);
+        const hipError_t ie = hipGraphInstantiate(&ch.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            ch.exec = nullptr;
+            ch.key.clear();
+            RTS_HIP(ie);
+        }
+        b->stats.graphs_built++;
+        RTS_HIP(hipGraphLaunch(ch.exec, s));
+    }
+    size_t resident = 0;
+    for (auto &kv : b->pipes) resident += shape_bytes(b, kv.first);
+    b->stats.arena_bytes = resident;
+    return RTS_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 rts_status rts_batch_create(const rts_pipeline_desc *base, size_t budget_bytes, int use_graphs, rts_batch **out) {
     RTS_CHECK(base && out, "null argument");
-    RTS_CHECK(base->cost1 == RTS_COST_RANK && base->cost2 == RTS_COST_LOG && base->cost3 == RTS_COST_NCC &&
-                  base->aggr == RTS_AGGR_BOX,
-              "the batch engine runs the rank + LoG + NCC cost with box aggregation");
-    RTS_CHECK(base->box_r >= 0 && base->box_r <= BQ_RMAX, "box radius %d must be in [0, %d]", base->box_r, BQ_RMAX);
-    RTS_CHECK(base->lambda1 > 0 && base->lambda2 > 0 && base->lambda3 > 0, "combine weights must be positive");
+    const bool special = base->cost1 == RTS_COST_RANK && base->cost2 == RTS_COST_LOG && base->cost3 == RTS_COST_NCC &&
+                         base->aggr == RTS_AGGR_BOX;
+    if (special) {
+        RTS_CHECK(base->box_r >= 0 && base->box_r <= BQ_RMAX, "box radius %d must be in [0, %d]", base->box_r,
+                  BQ_RMAX);
+        RTS_CHECK(base->lambda1 > 0 && base->lambda2 > 0 && base->lambda3 > 0, "combine weights must be positive");
+    } else {
+        // the configuration must make a pipeline (checked at a size every
+        // size check accepts; each pair's size is checked when it runs)
+        rts_pipeline_desc d = *base;
+        d.width = 64;
+        d.height = 64;
+        d.disp = 16;
+        if (int e = pipeline_check(d)) return e;
+    }
     RTS_CHECK(budget_bytes > 0, "empty budget");
     rts_batch *b = new (std::nothrow) rts_batch();
     RTS_CHECK(b != nullptr, "out of host memory");
     b->base = *base;
+    b->generic = !special;
     b->budget = budget_bytes;
     b->use_graphs = use_graphs != 0;
     *out = b;
@@ -453,6 +609,8 @@ rts_status rts_batch_create(const rts_pipeline_desc *base, size_t budget_bytes, 
 rts_status rts_batch_destroy(rts_batch *b) {
     if (!b) return RTS_OK;
     for (auto &c : b->chunks) free_chunk(c.get());
+    for (auto &c : b->pchunks) drop_pchunk(c.get());
+    for (auto &kv : b->pipes) rts_pipeline_destroy(kv.second);
     (void)hipFree(b->arena);
     delete b;
     return RTS_OK;
@@ -462,6 +620,16 @@ rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void 
     RTS_CHECK(b && (items || n == 0), "null argument");
     RTS_CHECK(n >= 0, "negative pair count");
     hipStream_t s = S(stream);
+    if (b->generic) {
+        for (int i = 0; i < n; i++) {
+            const rts_batch_item &it = items[i];
+            if (int e = check_dims(it.width, it.height, it.disp)) return e;
+            RTS_CHECK(it.left_rgb && it.right_rgb && it.dl && (!b->base.lr_check || (it.dr && it.mask)),
+                      "pair %d: null buffer", i);
+        }
+        RTS_CHECK(!b->use_graphs || stream != nullptr, "graph capture needs a stream other than the null stream");
+        return run_generic(b, items, n, s);
+    }
     for (int i = 0; i < n; i++) {
         const rts_batch_item &it = items[i];
         if (int e = check_dims(it.width, it.height, it.disp)) return e;

@@ -722,6 +722,16 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
     return RTS_OK;
 }
 
+namespace rts {
+int pipeline_check(const rts_pipeline_desc &d) { return validate(d); }
+
+int pipeline_enqueue(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb, uint8_t *dl, uint8_t *dr,
+                     uint8_t *mask, hipStream_t s) {
+    RTS_CHECK(p != nullptr, "null pipeline");
+    return pipeline_launch(p, left_rgb, right_rgb, nullptr, nullptr, nullptr, nullptr, dl, dr, mask, s);
+}
+}  // namespace rts
+
 extern "C" rts_status rts_pipeline_run(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb,
                                        const uint8_t *left_gray, const uint8_t *right_gray, const uint8_t *left_lab,
                                        const uint8_t *right_lab, uint8_t *dl, uint8_t *dr, uint8_t *mask,

@@ -5,6 +5,7 @@
 #include <stddef.h>
 
 #include "rtstereo.h"
+#include "sqrt_rn.h"
 
 namespace rts {
 
@@ -50,9 +51,11 @@ __device__ __forceinline__ int wrapn(int v, int n) {
 }
 
 // Correctly rounded float square root.  The hardware v_sqrt_f32 (which
-// __fsqrt_rn lowers to here) is not correctly rounded; a double square root
-// rounded once to float is, because 53 >= 2*24 + 2 bits.
-__device__ __forceinline__ float sqrt_rn(float x) { return (float)__builtin_sqrt((double)x); }
+// __fsqrt_rn lowers to here) is within one ulp but not correctly rounded;
+// rts_sqrt_fix moves its result to RN(sqrt(x)) with exact integer midpoint
+// tests (sqrt_rn.h; no double precision).
+__device__ __forceinline__ float hw_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ float sqrt_rn(float x) { return rts_sqrt_rn_with(x, hw_sqrt); }
 
 // Store of a volume element that the next kernel reads once: non-temporal,
 // so streamed volumes do not displace reusable lines from the caches.
@@ -107,6 +110,16 @@ int check_dims(int W, int H, int D);
 constexpr int WTA_MAX_ROWS = 65535;
 int wta_launch(const float *vol, uint8_t *dl, uint8_t *dr, uint8_t *mask, int W, int H, int D, int flags, int tol,
                int *carry, hipStream_t s);
+
+// rts_pipeline_create's checks of a configuration, without allocating.
+int pipeline_check(const rts_pipeline_desc &d);
+
+// One run's launch sequence of a pipeline on stream s, without graph
+// replay (colour images in; gray and Lab derived on the device): what
+// rts_pipeline_run launches for those buffers, for callers that capture the
+// launches of many pipelines into one graph (the batch engine).
+int pipeline_enqueue(rts_pipeline *p, const uint8_t *left_rgb, const uint8_t *right_rgb, uint8_t *dl, uint8_t *dr,
+                     uint8_t *mask, hipStream_t s);
 
 // Devices a process can address through per-device tables.
 constexpr int RTS_MAX_DEVICES = 64;

@@ -83,26 +83,32 @@ __global__ void k_rgb_to_gray(const uint8_t *__restrict__ rgb, uint8_t *__restri
     gray[i] = gray_bgr(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
 }
 
-// 8-bit CIELab (D65) from interleaved B,G,R bytes, in FP64: L*255/100,
-// a+128, b+128, rounded half to even and clipped.  srgb_lin[v] is the
-// linearised channel value v/255 (host table).
-__global__ void k_rgb_to_lab(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ lab,
-                             const double *__restrict__ srgb_lin, int n) {
+// 8-bit CIELab (D65) from interleaved B,G,R bytes, in FP32: sRGB
+// linearised, XYZ, then L*255/100, a+128, b+128, rounded half to even and
+// clipped.  (The bytes equal an FP64 evaluation except where a value lies
+// within FP32 rounding of a half-integer: about 1.5e-5 of them, by one level.)
+__device__ __forceinline__ float srgb_lin(uint8_t v) {
+    const float c = (float)v / 255.0f;
+    return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f);
+}
+
+__device__ __forceinline__ float lab_f(float t) {
+    const float eps = 216.0f / 24389.0f, kap = 24389.0f / 27.0f;
+    return t > eps ? cbrtf(t) : (kap * t + 16.0f) / 116.0f;
+}
+
+__global__ void k_rgb_to_lab(const uint8_t *__restrict__ rgb, uint8_t *__restrict__ lab, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double b = srgb_lin[rgb[3 * i]], g = srgb_lin[rgb[3 * i + 1]], r = srgb_lin[rgb[3 * i + 2]];
-    double X = 0.4124564 * r + 0.3575761 * g + 0.1804375 * b;
-    double Y = 0.2126729 * r + 0.7151522 * g + 0.0721750 * b;
-    double Z = 0.0193339 * r + 0.1191920 * g + 0.9503041 * b;
-    X /= 0.950456;
-    Z /= 1.088754;
-    const double eps = 216.0 / 24389.0, kap = 24389.0 / 27.0;
-    const double fx = X > eps ? cbrt(X) : (kap * X + 16.0) / 116.0;
-    const double fy = Y > eps ? cbrt(Y) : (kap * Y + 16.0) / 116.0;
-    const double fz = Z > eps ? cbrt(Z) : (kap * Z + 16.0) / 116.0;
-    const double v[3] = {(116.0 * fy - 16.0) * 255.0 / 100.0, 500.0 * (fx - fy) + 128.0, 200.0 * (fy - fz) + 128.0};
+    const float b = srgb_lin(rgb[3 * i]), g = srgb_lin(rgb[3 * i + 1]), r = srgb_lin(rgb[3 * i + 2]);
+    const float X = (0.4124564f * r + 0.3575761f * g + 0.1804375f * b) / 0.950456f;
+    const float Y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
+    const float Z = (0.0193339f * r + 0.1191920f * g + 0.9503041f * b) / 1.088754f;
+    const float fx = lab_f(X), fy = lab_f(Y), fz = lab_f(Z);
+    const float v[3] = {(116.0f * fy - 16.0f) * 255.0f / 100.0f, 500.0f * (fx - fy) + 128.0f,
+                        200.0f * (fy - fz) + 128.0f};
 #pragma unroll
-    for (int c = 0; c < 3; c++) lab[3 * i + c] = (uint8_t)fmin(255.0, fmax(0.0, rint(v[c])));
+    for (int c = 0; c < 3; c++) lab[3 * i + c] = (uint8_t)fminf(fmaxf(__builtin_rintf(v[c]), 0.0f), 255.0f);
 }
 
 // 3x3 median of each colour channel with replicated borders (the OpenCL
@@ -277,27 +283,8 @@ rts_status rts_rgb_to_gray(const uint8_t *rgb, uint8_t *gray, int W, int H, void
 
 rts_status rts_rgb_to_lab(const uint8_t *rgb, uint8_t *lab, int W, int H, void *stream) {
     RTS_CHECK(W >= 1 && H >= 1 && (long)W * H < (1L << 31), "bad image size %dx%d", W, H);
-    static double *table[64] = {};   // per device: the 256 linearised sRGB values
-    int dev = 0;
-    RTS_HIP(hipGetDevice(&dev));
-    RTS_CHECK(dev >= 0 && dev < 64, "device %d out of range", dev);
-    static std::mutex mu;
-    {
-        std::lock_guard<std::mutex> g(mu);
-        if (!table[dev]) {
-            double h[256];
-            for (int v = 0; v < 256; v++) {
-                const double c = v / 255.0;
-                h[v] = c <= 0.04045 ? c / 12.92 : pow((c + 0.055) / 1.055, 2.4);
-            }
-            double *d = nullptr;
-            RTS_HIP(hipMalloc((void **)&d, sizeof(h)));
-            RTS_HIP(hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice));
-            table[dev] = d;
-        }
-    }
     const int n = W * H;
-    k_rgb_to_lab<<<div_up(n, 256), 256, 0, S(stream)>>>(rgb, lab, table[dev], n);
+    k_rgb_to_lab<<<div_up(n, 256), 256, 0, S(stream)>>>(rgb, lab, n);
     RTS_LAUNCHED();
     return RTS_OK;
 }

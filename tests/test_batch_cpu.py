@@ -9,9 +9,15 @@ from rtstereo_amd import batch
 from rtstereo_amd.pipeline import preset
 
 
-def test_batch_rejects_other_configurations():
+def test_batch_configurations():
+    # other pipeline configurations run through resident per-shape pipelines
+    for name, over in (("census_cross", {}), ("aw", {"aw_precision": L.RTS_AW_FP16})):
+        batch.Batch(preset(name, 64, 48, 16, **over), 1 << 30).close()
+    # ... but only valid ones
+    bad = preset("census_cross", 64, 48, 16)
+    bad.cost1, bad.cost2 = L.RTS_COST_NCC, 0
     with pytest.raises(RtsError) as e:
-        batch.Batch(preset("census_cross", 64, 48, 16), 1 << 30)
+        batch.Batch(bad, 1 << 30)
     assert e.value.status == L.RTS_ERR_INVALID
     d = preset("rank_log_ncc", 64, 48, 16)
     d.box_r = 9

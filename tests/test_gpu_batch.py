@@ -60,3 +60,39 @@ def test_batch_chunks_and_eager_equal_graphs():
     assert st.chunks >= 3 and st2.chunks == 1 and st2.graphs_built == 0
     for (a, b_, c), (x, y, z) in zip(g[0], e[0]):
         assert np.array_equal(a, x) and np.array_equal(b_, y) and np.array_equal(c, z)
+
+
+MIXED = [(96, 64, 16), (160, 96, 40), (132, 70, 72), (96, 64, 16), (200, 60, 130), (160, 96, 40)]
+
+
+@pytest.mark.parametrize("name,over", [("aw", {"aw_precision": 1}), ("census_cross", {}),
+                                       ("census_cross", {"wta_literal": 1, "cross_iters": 2})])
+def test_pipeline_batch_matches_pipelines(name, over):
+    """Adaptive-weight FP16 (33x33) and census + cross pairs of mixed sizes
+    through the batch engine's pipeline path: every map equals the pair run
+    alone through its own pipeline, in one chunk (captured, then replayed)
+    and under a budget that forces several chunks and pipeline eviction."""
+    print(f"[case] pipeline batch {name} {over}", flush=True)
+    pairs = [gen.make_pair(W, H, D, 6, 2.5, 80 + i) for i, (W, H, D) in enumerate(MIXED)]
+    want = [PL.run_host(PL.preset(name, W, H, D, **over), p.left, p.right)[:3] for p, (W, H, D) in zip(pairs, MIXED)]
+    s = Stream()
+    dev = [B.Pair(p.left, p.right, D) for p, (_, _, D) in zip(pairs, MIXED)]
+    need = max(B.pipeline_bytes(PL.preset(name, W, H, D, **over)) for W, H, D in MIXED)
+    for budget, reps in ((1 << 34, 2), (int(need * 1.5), 2)):
+        b = B.Batch(PL.preset(name, 64, 64, 16, **over), budget, True)
+        for _ in range(reps):
+            for d in dev:
+                for a in (d.dl, d.dr, d.mask):
+                    a.fill(77)
+            b.run(dev, s.handle)
+            s.sync()
+            for d, w in zip(dev, want):
+                for g, x in zip(d.results(), w):
+                    assert np.array_equal(g, x)
+        st = b.stats()
+        assert st.graphs_built >= 1 and (budget < (1 << 34) or (st.chunks == 1 and st.graph_replays == 1))
+        if budget < (1 << 34):
+            assert st.chunks >= 2
+        b.close()
+    for d in dev:
+        d.free()
