@@ -385,14 +385,22 @@ typedef struct rts_batch_stats {
     int graphs_built, graph_replays;        /* cumulative */
     unsigned long long box_items;           /* cumulative box work items */
     size_t arena_bytes;
-    int box_rows, cost_disparities;         /* tuned tile shape */
-    float tune_ms[6];                       /* cost dch 8/16/32, box rows 32 / 64-2r rounded
-                                               down to a multiple of 4 / 64 */
+    int box_rows, box_cols;                 /* tuned box tile */
+    int cost_disparities;                   /* tuned disparities per cost block */
+    float tune_ms[9];                       /* cost dch 8/16/32; box tiles 64 x (32, tb, 64)
+                                               and 128 x (32, tb, 64), tb = 64-2r rounded
+                                               down to a multiple of 4 */
 } rts_batch_stats;
 typedef struct rts_batch rts_batch;
 rts_status rts_batch_create(const rts_pipeline_desc *base, size_t budget_bytes, int use_graphs, rts_batch **out);
 rts_status rts_batch_destroy(rts_batch *b);
 rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void *stream);
+/* Fixes the rank + LoG + NCC path's tuned parameters instead of timing the
+ * candidates on the first chunk: box tile of box_cols (64 or 128) x
+ * box_rows (32, 64, or 64 - 2r rounded down to a multiple of 4) and
+ * cost_disparities (8, 16 or 32) per cost block; all three 0 returns to
+ * tuning on the next run.  The results do not depend on these choices. */
+rts_status rts_batch_set_tiles(rts_batch *b, int box_cols, int box_rows, int cost_disparities);
 rts_status rts_batch_get_stats(rts_batch *b, rts_batch_stats *st);
 size_t rts_batch_pair_bytes(int W, int H, int D);
 

@@ -23,13 +23,14 @@ class BatchItem(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("pairs", C.c_int), ("chunks", C.c_int), ("graphs_built", C.c_int), ("graph_replays", C.c_int),
                 ("box_items", C.c_ulonglong), ("arena_bytes", C.c_size_t), ("box_rows", C.c_int),
-                ("cost_disparities", C.c_int), ("tune_ms", C.c_float * 6)]
+                ("box_cols", C.c_int), ("cost_disparities", C.c_int), ("tune_ms", C.c_float * 9)]
 
 
 _SIGS = {
     "rts_batch_create": ([C.POINTER(PipelineDesc), C.c_size_t, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "rts_batch_destroy": ([C.c_void_p], C.c_int),
     "rts_batch_run": ([C.c_void_p, C.POINTER(BatchItem), C.c_int, C.c_void_p], C.c_int),
+    "rts_batch_set_tiles": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
     "rts_batch_get_stats": ([C.c_void_p, C.POINTER(BatchStats)], C.c_int),
     "rts_batch_pair_bytes": ([C.c_int, C.c_int, C.c_int], C.c_size_t),
 }
@@ -80,6 +81,10 @@ class Batch:
     def run(self, pairs, stream) -> None:
         arr = (BatchItem * len(pairs))(*[p.item() for p in pairs])
         call("rts_batch_run", self.handle, arr, len(pairs), stream)
+
+    def set_tiles(self, box_cols: int, box_rows: int, cost_disparities: int) -> None:
+        """Fix the tuned tile shapes (all three 0: tune again on the next run)."""
+        call("rts_batch_set_tiles", self.handle, box_cols, box_rows, cost_disparities)
 
     def stats(self) -> BatchStats:
         st = BatchStats()
@@ -188,7 +193,7 @@ def main(argv=None) -> int:
            "chunks": st.chunks, "device_bytes": st.arena_bytes, "graphs_built": st.graphs_built,
            "graph_replays": st.graph_replays, "input_generation_s": gen_s, "graphs": not args.no_graphs}
     if args.config == "rank_log_ncc":
-        out["tuned"] = {"box_rows": st.box_rows, "cost_disparities": st.cost_disparities,
+        out["tuned"] = {"box_rows": st.box_rows, "box_cols": st.box_cols, "cost_disparities": st.cost_disparities,
                         "tune_ms": [round(x, 3) for x in st.tune_ms]}
     print(json.dumps(out))
     return 0

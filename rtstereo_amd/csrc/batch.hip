@@ -58,16 +58,24 @@ struct BoxItem {
     bool inner;   // the tile's window lies inside the image (no wrap)
 };
 
-template <int TY, int R>
+// A work item is a TX x TY tile of one plane (TX = 64 or 128 columns): its
+// (TY + 2R) x (TX + 2R) window is staged in LDS, so the pair (TX, TY) sets
+// the block's LDS footprint (with the 12 KB pair table, for r = 5: 34 KB at
+// 64 x 32, 52 KB at 64 x 64, 90 KB at 128 x 64) and, with the registers the
+// tile's window takes, the blocks per CU (4 down to 1).
+template <int TX, int TY, int R>
 __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ pairs, int npairs, unsigned ntiles,
                                                    unsigned *__restrict__ counter) {
-    constexpr int E = 2 * R + 1, TH = TY + 2 * R, TW = 64 + 2 * R;
+    constexpr int E = 2 * R + 1, TH = TY + 2 * R, TW = TX + 2 * R;
     constexpr int NB = (TH * TW + 255) / 256;
+    constexpr int NG = 256 / TX;   // row groups of the vertical pass
+    static_assert(TX == 64 || TX == 128, "tile width");
+    static_assert(TY % NG == 0, "strip rows must split over the row groups");
     __shared__ float tile[TH][TW + 1];
-    __shared__ float hs[TH][65];
+    __shared__ float hs[TH][TX + 1];
     __shared__ BoxPair sp[BQ_MAXP];   // the pair table: decoding an item reads no global memory
     __shared__ unsigned item;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int tx = threadIdx.x & (TX - 1), ty = threadIdx.x / TX;
     for (int i = threadIdx.x; i < npairs; i += 256) sp[i] = pairs[i];
     auto decode = [&](unsigned t) {
         int lo = 0, hi = npairs - 1;   // last pair whose first item is <= t
@@ -85,9 +93,9 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
         q.dst = bp.out + base;
         q.W = bp.W;
         q.H = bp.H;
-        q.x0 = xb * 64;
+        q.x0 = xb * TX;
         q.y0 = sb * TY;
-        q.inner = q.x0 - R >= 0 && q.x0 + 64 + R <= q.W && q.y0 - R >= 0 && q.y0 + TY + R <= q.H;
+        q.inner = q.x0 - R >= 0 && q.x0 + TX + R <= q.W && q.y0 - R >= 0 && q.y0 + TY + R <= q.H;
         return q;
     };
     float v[NB];
@@ -141,8 +149,8 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
         // Horizontal sums: a thread takes a 16-column piece of one row, keeps
         // its 16 + 2R inputs in registers and adds each output's E taps in
         // column order (one LDS read per input instead of E per output).
-        for (int task = threadIdx.x; task < TH * 4; task += 256) {
-            const int rr = task >> 2, c0 = (task & 3) * 16;
+        for (int task = threadIdx.x; task < TH * (TX / 16); task += 256) {
+            const int rr = task / (TX / 16), c0 = (task % (TX / 16)) * 16;
             float w[16 + 2 * R];
 #pragma unroll
             for (int i = 0; i < 16 + 2 * R; i++) w[i] = tile[rr][c0 + i];
@@ -155,10 +163,10 @@ __global__ void __launch_bounds__(256) k_box_queue(const BoxPair *__restrict__ p
             }
         }
         __syncthreads();
-        // Vertical sums: a thread takes TY/4 consecutive output rows of one
-        // column, its TY/4 + 2R horizontal sums in registers, taps in row
+        // Vertical sums: a thread takes TY/NG consecutive output rows of one
+        // column, its TY/NG + 2R horizontal sums in registers, taps in row
         // order.
-        constexpr int RPT = TY / 4;
+        constexpr int RPT = TY / NG;
         const int x = cur.x0 + tx;
         if (x < cur.W) {
             float cv[RPT + 2 * R];
@@ -201,7 +209,8 @@ struct Chunk {
     size_t dpairs_cap = 0;
     unsigned *counter = nullptr;
     unsigned ntiles = 0;
-    int box_ty = 0;
+    int box_tx = 0, box_ty = 0;
+    int cost_dch = 0;   // disparities per cost block of the captured graph
     hipGraphExec_t exec = nullptr;
 };
 
@@ -243,8 +252,8 @@ struct rts_batch {
     uint8_t *arena = nullptr;
     size_t arena_bytes = 0;
     int use_graphs = 1;
-    int tuned = 0, box_ty = 32, cost_dch = 16;
-    float tune_ms[6] = {};
+    int tuned = 0, box_tx = 64, box_ty = 32, cost_dch = 16;
+    float tune_ms[9] = {};
     std::vector<std::unique_ptr<Chunk>> chunks;
     rts_batch_stats stats{};
 };
@@ -266,38 +275,49 @@ void free_chunk(Chunk *c) {
 // 64 - 2R rounded down to a multiple of 4 (52 rows for the 11x11 box).
 constexpr int box_balanced_rows(int r) { return ((64 - 2 * r) / 4) * 4; }
 
-template <int R>
-void launch_box_r(const Chunk &c, int ty, int nb, hipStream_t s) {
+template <int TX, int R>
+void launch_box_w(const Chunk &c, int ty, int nb, hipStream_t s) {
     constexpr int TB = box_balanced_rows(R);
     if (ty == 32)
-        k_box_queue<32, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
+        k_box_queue<TX, 32, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
     else if (ty == TB)
-        k_box_queue<TB, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
+        k_box_queue<TX, TB, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
     else
-        k_box_queue<64, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
+        k_box_queue<TX, 64, R><<<nb, 256, 0, s>>>(c.dpairs, (int)c.bufs.size(), c.ntiles, c.counter);
 }
 
-int launch_box(const Chunk &c, int ty, int r, hipStream_t s) {
+template <int R>
+void launch_box_r(const Chunk &c, int tx, int ty, int nb, hipStream_t s) {
+    if (tx == 128)
+        launch_box_w<128, R>(c, ty, nb, s);
+    else
+        launch_box_w<64, R>(c, ty, nb, s);
+}
+
+// The box over the chunk's work table, built for tiles of tx x ty.
+int launch_box(const Chunk &c, int r, hipStream_t s) {
+    const int tx = c.box_tx, ty = c.box_ty;
     const int nb = std::max(1, std::min<int>((int)std::min<unsigned>(c.ntiles, 1u << 30), 256 * 8));
     k_zero_u32<<<1, 64, 0, s>>>(c.counter, 1);
     RTS_LAUNCHED();
     switch (r) {
-        case 0: launch_box_r<0>(c, ty, nb, s); break;
-        case 1: launch_box_r<1>(c, ty, nb, s); break;
-        case 2: launch_box_r<2>(c, ty, nb, s); break;
-        case 3: launch_box_r<3>(c, ty, nb, s); break;
-        case 4: launch_box_r<4>(c, ty, nb, s); break;
-        case 5: launch_box_r<5>(c, ty, nb, s); break;
-        case 6: launch_box_r<6>(c, ty, nb, s); break;
-        case 7: launch_box_r<7>(c, ty, nb, s); break;
-        default: launch_box_r<BQ_RMAX>(c, ty, nb, s); break;
+        case 0: launch_box_r<0>(c, tx, ty, nb, s); break;
+        case 1: launch_box_r<1>(c, tx, ty, nb, s); break;
+        case 2: launch_box_r<2>(c, tx, ty, nb, s); break;
+        case 3: launch_box_r<3>(c, tx, ty, nb, s); break;
+        case 4: launch_box_r<4>(c, tx, ty, nb, s); break;
+        case 5: launch_box_r<5>(c, tx, ty, nb, s); break;
+        case 6: launch_box_r<6>(c, tx, ty, nb, s); break;
+        case 7: launch_box_r<7>(c, tx, ty, nb, s); break;
+        default: launch_box_r<BQ_RMAX>(c, tx, ty, nb, s); break;
     }
     RTS_LAUNCHED();
     return RTS_OK;
 }
 
-// Sets the chunk's box work table for strip height ty (device upload).
-int build_box_table(Chunk &c, const rts_batch_item *items, int ty, hipStream_t s) {
+// Sets the chunk's box work table for tiles of tx columns x ty rows (device
+// upload).
+int build_box_table(Chunk &c, const rts_batch_item *items, int tx, int ty, hipStream_t s) {
     std::vector<BoxPair> t(c.items.size());
     unsigned n = 0;
     for (size_t i = 0; i < c.items.size(); i++) {
@@ -308,7 +328,7 @@ int build_box_table(Chunk &c, const rts_batch_item *items, int ty, hipStream_t s
         b.W = it.width;
         b.H = it.height;
         b.D = it.disp;
-        b.nx = div_up(it.width, 64);
+        b.nx = div_up(it.width, tx);
         b.ny = div_up(it.height, ty);
         b.tile0 = n;
         const unsigned long add = (unsigned long)b.nx * b.ny * b.D;
@@ -326,6 +346,7 @@ int build_box_table(Chunk &c, const rts_batch_item *items, int ty, hipStream_t s
     RTS_HIP(hipMemcpyAsync(c.dpairs, t.data(), sizeof(BoxPair) * t.size(), hipMemcpyHostToDevice, s));
     RTS_HIP(hipStreamSynchronize(s));
     c.ntiles = n;
+    c.box_tx = tx;
     c.box_ty = ty;
     return RTS_OK;
 }
@@ -357,7 +378,7 @@ int enqueue_chunk(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream
         if (int e = prep_pair(b, it, c.bufs[i], s)) return e;
         if (int e = cost_pair(b, it, c.bufs[i], b->cost_dch, s)) return e;
     }
-    if (int e = launch_box(c, c.box_ty, b->base.box_r, s)) return e;
+    if (int e = launch_box(c, b->base.box_r, s)) return e;
     for (size_t i = 0; i < c.items.size(); i++) {
         const rts_batch_item &it = items[c.items[i]];
         if (int e = rts_wta(c.bufs[i].agg, it.dl, it.dr, it.mask, it.width, it.height, it.disp, 0, b->base.lr_tol, s))
@@ -394,19 +415,24 @@ void carve(rts_batch *b, Chunk &c, const rts_batch_item *items) {
 
 // Times the candidates of each tuned parameter over every pair of the first
 // chunk (the best of the timed runs after a warm-up run): the cost kernel's
-// disparities per block (8, 16, 32), one launch per pair, and the box strip
-// height (32 rows, the balanced height of box_balanced_rows, 64 rows), one
-// persistent launch.  The chunk's mix matters:
-// on the largest pair the candidates nearly tie, while larger tiles leave
-// the small pairs' launches with few blocks.  A larger candidate replaces a
-// smaller one only when it is 2% faster.
+// disparities per block (8, 16, 32), one launch per pair, and the box tile:
+// 64 or 128 columns by a strip of 32 rows, the balanced height of
+// box_balanced_rows or 64 rows, one persistent launch.  The tile sets the
+// LDS footprint (for r = 5 from 34 KB at 64 x 32 to 90 KB at 128 x 64) and so
+// the blocks per CU, and the number of work items.  The chunk's mix matters:
+// on the largest pair the cost candidates nearly tie, while larger tiles
+// leave the small pairs' launches with few blocks.  Candidates are tried
+// from the smallest footprint up; a larger one replaces a smaller one only
+// when it is 2% faster.
 int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s) {
     hipEvent_t e0, e1;
     RTS_HIP(hipEventCreate(&e0));
     RTS_HIP(hipEventCreate(&e1));
     int st = RTS_OK;
     for (size_t i = 0; i < c.items.size() && st == RTS_OK; i++) st = prep_pair(b, items[c.items[i]], c.bufs[i], s);
-    const int dchs[3] = {8, 16, 32}, tys[3] = {32, box_balanced_rows(b->base.box_r), 64};
+    const int dchs[3] = {8, 16, 32};
+    const int tb = box_balanced_rows(b->base.box_r);
+    const int box_tx[6] = {64, 64, 64, 128, 128, 128}, box_ty[6] = {32, tb, 64, 32, tb, 64};
     constexpr int REPS = 3;            // the first run warms up
     constexpr float SWITCH = 0.98f;    // a larger candidate must be 2% faster
     // times run(k) as the best of REPS - 1 runs after a warm-up
@@ -436,13 +462,15 @@ int autotune(rts_batch *b, Chunk &c, const rts_batch_item *items, hipStream_t s)
         }
     }
     best = 1e30f;
-    for (int k = 0; k < 3 && st == RTS_OK; k++) {
-        st = build_box_table(c, items, tys[k], s);
-        const float ms = time_it([&]() { return launch_box(c, tys[k], b->base.box_r, s); });
+    for (int k = 0; k < 6 && st == RTS_OK; k++) {
+        st = build_box_table(c, items, box_tx[k], box_ty[k], s);
+        if (st != RTS_OK) break;
+        const float ms = time_it([&]() { return launch_box(c, b->base.box_r, s); });
         b->tune_ms[3 + k] = ms;
         if (k == 0 || ms < best * SWITCH) {
             best = ms;
-            b->box_ty = tys[k];
+            b->box_tx = box_tx[k];
+            b->box_ty = box_ty[k];
         }
     }
     (void)hipEventDestroy(e0);
@@ -673,7 +701,8 @@ rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void 
     b->stats.arena_bytes = b->arena_bytes;
     for (size_t k = 0; k < plan.size(); k++) {
         Chunk &c = *b->chunks[k];
-        const bool reuse = c.exec && c.box_ty == b->box_ty && same_items(c.key, items, plan[k]);
+        const bool reuse = c.exec && c.box_tx == b->box_tx && c.box_ty == b->box_ty && c.cost_dch == b->cost_dch &&
+                           same_items(c.key, items, plan[k]);
         if (!reuse) {
             if (c.exec) (void)hipGraphExecDestroy(c.exec);
             c.exec = nullptr;
@@ -683,7 +712,8 @@ rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void 
             carve(b, c, items);
             if (!b->tuned)
                 if (int e = autotune(b, c, items, s)) return e;
-            if (int e = build_box_table(c, items, b->box_ty, s)) return e;
+            if (int e = build_box_table(c, items, b->box_tx, b->box_ty, s)) return e;
+            c.cost_dch = b->cost_dch;
             if (b->use_graphs) {
                 hipGraph_t g = nullptr;
                 RTS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -710,8 +740,28 @@ rts_status rts_batch_run(rts_batch *b, const rts_batch_item *items, int n, void 
         b->stats.box_items += c.ntiles;
     }
     b->stats.box_rows = b->box_ty;
+    b->stats.box_cols = b->box_tx;
     b->stats.cost_disparities = b->cost_dch;
-    for (int i = 0; i < 6; i++) b->stats.tune_ms[i] = b->tune_ms[i];
+    for (int i = 0; i < 9; i++) b->stats.tune_ms[i] = b->tune_ms[i];
+    return RTS_OK;
+}
+
+rts_status rts_batch_set_tiles(rts_batch *b, int box_cols, int box_rows, int cost_disparities) {
+    RTS_CHECK(b != nullptr, "null batch");
+    if (box_cols == 0 && box_rows == 0 && cost_disparities == 0) {
+        b->tuned = 0;   // time the candidates again on the next run's first chunk
+        return RTS_OK;
+    }
+    const int tb = box_balanced_rows(b->base.box_r);
+    RTS_CHECK(box_cols == 64 || box_cols == 128, "box tile width %d must be 64 or 128", box_cols);
+    RTS_CHECK(box_rows == 32 || box_rows == tb || box_rows == 64, "box strip of %d rows must be 32, %d or 64", box_rows,
+              tb);
+    RTS_CHECK(cost_disparities == 8 || cost_disparities == 16 || cost_disparities == 32,
+              "disparities per cost block %d must be 8, 16 or 32", cost_disparities);
+    b->box_tx = box_cols;
+    b->box_ty = box_rows;
+    b->cost_dch = cost_disparities;
+    b->tuned = 1;
     return RTS_OK;
 }
 

@@ -25,6 +25,16 @@ def test_batch_configurations():
         batch.Batch(d, 1 << 30)
 
 
+def test_set_tiles_checks():
+    b = batch.Batch(preset("rank_log_ncc", 64, 48, 16), 1 << 30)
+    b.set_tiles(128, 52, 8)          # 52: 64 - 2r rounded down to a multiple of 4, r = 5
+    b.set_tiles(0, 0, 0)
+    for bad in ((96, 32, 16), (64, 48, 16), (64, 32, 12)):
+        with pytest.raises(RtsError):
+            b.set_tiles(*bad)
+    b.close()
+
+
 def test_pair_bytes_and_budget():
     W, H, D = 640, 480, 64
     n = batch.pair_bytes(W, H, D)

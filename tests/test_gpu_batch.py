@@ -19,10 +19,12 @@ def _pairs(seed=50):
     return [gen.make_pair(W, H, D, 6, 2.5, seed + i) for i, (W, H, D) in enumerate(SHAPES)]
 
 
-def _run(pairs, budget, graphs, reps=1):
+def _run(pairs, budget, graphs, reps=1, tiles=None):
     s = Stream()
     dev = [B.Pair(p.left, p.right, D) for p, (_, _, D) in zip(pairs, SHAPES)]
     b = B.Batch(PL.preset("rank_log_ncc", 64, 64, 16), budget, graphs)
+    if tiles:
+        b.set_tiles(*tiles)
     outs = []
     for _ in range(reps):
         b.run(dev, s.handle)
@@ -40,7 +42,8 @@ def test_batch_matches_oracle_and_pipeline():
     pairs = _pairs()
     outs, st = _run(pairs, 1 << 30, True, reps=2)
     assert st.chunks == 1 and st.graphs_built == 1 and st.graph_replays == 1
-    assert st.box_rows in (32, 52, 64) and st.cost_disparities in (8, 16, 32)   # 52: balanced rows for r = 5
+    assert st.box_rows in (32, 52, 64) and st.box_cols in (64, 128)   # 52: balanced rows for r = 5
+    assert st.cost_disparities in (8, 16, 32) and min(st.tune_ms) > 0
     for (p, (W, H, D)), (dl, dr, m), (dl2, dr2, m2) in zip(zip(pairs, SHAPES), outs[0], outs[1]):
         assert np.array_equal(dl, dl2) and np.array_equal(m, m2)            # replay reproduces the capture
         desc = PL.preset("rank_log_ncc", W, H, D)
@@ -49,6 +52,22 @@ def test_batch_matches_oracle_and_pipeline():
         # separable box sums round differently from the row-major window sum
         assert (dl != odl).mean() <= 1e-3 and (dr != odr).mean() <= 1e-3
         assert (dl != pdl).mean() <= 1e-3
+
+
+@pytest.mark.parametrize("tiles", [(64, 32, 8), (64, 52, 16), (64, 64, 32), (128, 32, 16), (128, 52, 8),
+                                   (128, 64, 16)])
+def test_batch_tiles_do_not_change_results(tiles):
+    """Every box tile (64 or 128 columns by 32, 52 or 64 rows, which sets the
+    LDS footprint) and cost block size the tuner may pick gives the same maps
+    as the tuned run: each output is summed in the same order whatever the
+    tile."""
+    print(f"[case] batch 6 mixed pairs, fixed tiles {tiles}", flush=True)
+    pairs = _pairs(70)
+    ref, _ = _run(pairs, 1 << 30, True)
+    got, st = _run(pairs, 1 << 30, True, tiles=tiles)
+    assert (st.box_cols, st.box_rows, st.cost_disparities) == tiles
+    for (dl, dr, m), (dl2, dr2, m2) in zip(ref[0], got[0]):
+        assert np.array_equal(dl, dl2) and np.array_equal(dr, dr2) and np.array_equal(m, m2)
 
 
 def test_batch_chunks_and_eager_equal_graphs():
