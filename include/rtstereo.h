@@ -179,6 +179,13 @@ rts_status rts_cost_rank_log_ncc_ws(const uint8_t *gray_ref, const uint8_t *gray
  * (whose in-place update is replaced by a true box filter). */
 rts_status rts_box(const float *in, float *out, int W, int H, int D, int r, void *stream);
 
+/* rts_box for integer-valued volumes, by running sums (O(1) adds per output;
+ * r <= 16).  When every input is an integer in [0, 2^24 / (2r+1)^2] (AD of
+ * bytes, AD truncated at an integer, census Hamming, rank AD) every partial
+ * window sum is exact and the output equals rts_box's bit for bit; other
+ * inputs give undefined values.  seq/ConstBlockAggregation.cpp:11-34. */
+rts_status rts_box_int(const float *in, float *out, int W, int H, int D, int r, void *stream);
+
 /* Proximity weights exp(-sqrt(dy^2+dx^2)/gp) into a device float[(2r+1)^2].
  * seq/AdaptiveWeightAggregation.cpp:11-18 calcProxWeight. */
 rts_status rts_prox_weight(float *prox, float gamma_p, int r, void *stream);

@@ -74,6 +74,7 @@ _SIGS = {
     "rts_ncc": [vp, vp, vp, i32, i32, i32, i32, i32, i32, vp],
     "rts_combine": [vp, vp, f32, f32, sz, vp],
     "rts_box": [vp, vp, i32, i32, i32, i32, vp],
+    "rts_box_int": [vp, vp, i32, i32, i32, i32, vp],
     "rts_prox_weight": [vp, f32, i32, vp],
     "rts_compute_weights": [vp, vp, vp, f32, i32, i32, i32, vp],
     "rts_aw_weights": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],

@@ -5,6 +5,7 @@
 
 #include <stdlib.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #pragma clang fp contract(off)
@@ -132,6 +133,120 @@ __global__ void __launch_bounds__(256) k_box_f32_global(const float *__restrict_
         for (int dx = -r; dx <= r; dx++) acc += in[rr + wrap1(x + dx, W)];
     }
     out[base + (size_t)y * W + x] = __fdiv_rn(acc, (float)((2 * r + 1) * (2 * r + 1)));
+}
+
+// Box over integer-valued costs by running sums (seq/ConstBlockAggregation.cpp:11-34
+// for volumes whose every value is an integer in [0, 2^24 / (2r+1)^2]: AD of
+// bytes, truncated AD at an integer threshold, census Hamming, rank AD).
+// Every partial sum of such a window is an exact integer, so the float sum
+// the reference forms in row-major order equals the integer window sum, and
+// the quotient by the area is the same float.
+//
+// A wave owns one d plane, an aligned tile of 64 output columns x0 .. x0+63
+// and a segment of rows.  Per input row it loads the tile's 64 columns and,
+// on lanes 0 .. r-1 and 64-r .. 63, the 2r halo columns beside it (wrapped),
+// forms the inclusive prefix of each across the wave (DPP row shifts, then
+// the row broadcasts), and takes each output's row sum h from prefix values
+// (ds_bpermute): the tile part S[min(j+r,63)] - S[j-r-1], plus the part of
+// the left halo at or right of column j-r, plus the part of the right halo up
+// to column j+r.  The column sum runs down the segment:
+// V(y+1) = V(y) + h(y+r+1) - h(y-r), the last 2r+1 row sums kept in a
+// per-wave LDS ring.  O(1) adds per output instead of (2r+1)^2; loads and
+// stores are whole aligned rows of the tile.  Waves are independent (no
+// block barrier).
+constexpr int BOXI_RMAX = 16;
+constexpr int BOXI_PF = 8;   // input rows in flight per wave
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15 into rows 1, 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31 into rows 2, 3
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_box_int(const float *__restrict__ in, float *__restrict__ out, int W, int H,
+                                                 int r, int ntx, int nseg, int seg, int nwork) {
+    extern __shared__ uint32_t boxi_ring[];   // per wave: 2r+1 rows of 64 row sums
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int work = blockIdx.x * 4 + wv;
+    if (work >= nwork) return;   // the whole wave leaves
+    const int xt = work % ntx, rest = work / ntx;
+    const int sg = rest % nseg, d = rest / nseg;
+    const int E = 2 * r + 1;
+    const int x0 = xt * 64, x = x0 + lane;
+    const bool outlane = x < W;
+    const int cmain = wrapn(x, W);
+    // halo lanes: j < r holds column x0-r+j, j >= 64-r holds column x0+j+r
+    const bool lhalo = lane < r, rhalo = lane >= 64 - r;
+    const int chalo = lhalo ? wrapn(x0 - r + lane, W) : rhalo ? wrapn(x0 + lane + r, W) : cmain;
+    const int s0 = sg * seg, s1 = min(H, s0 + seg);
+    const size_t base = (size_t)d * plane(W, H);
+    const float *src = in + base;
+    float *dst = out + base;
+    uint32_t(*rg)[64] = reinterpret_cast<uint32_t(*)[64]>(boxi_ring + (size_t)wv * E * 64);
+    const int a_hi = min(lane + r, 63) * 4, a_lo = ((lane - r - 1) & 63) * 4, a_prev = ((lane - 1) & 63) * 4;
+    const bool has_lo = lane > r;
+    auto load = [&](int y) {
+        const float *row = src + (size_t)wrapn(y, H) * W;
+        return make_float2(row[cmain], (lhalo || rhalo) ? row[chalo] : 0.0f);
+    };
+    // window sum over the output's 2r+1 columns of one input row
+    auto hrow = [&](float2 v) -> uint32_t {
+        const uint32_t S = wave_incl_scan((uint32_t)v.x);
+        const uint32_t U = wave_incl_scan((uint32_t)v.y);   // zero between the two halos
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(a_hi, (int)S);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(a_lo, (int)S);
+        const uint32_t up = (uint32_t)__builtin_amdgcn_ds_bpermute(a_prev, (int)U);
+        const uint32_t ul = r > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)U, r - 1) : 0u;   // the left halo's sum
+        uint32_t h = hi - (has_lo ? lo : 0u);
+        if (lhalo) h += ul - (lane > 0 ? up : 0u);   // left halo columns j-r .. x0-1
+        if (rhalo) h += U - ul;                       // right halo columns x0+64 .. j+r
+        return h;
+    };
+    const uint32_t area = (uint32_t)(E * E);
+    // the first window: rows s0-r .. s0+r into ring slots 0 .. 2r
+    uint32_t V = 0;
+    for (int i0 = 0; i0 < E; i0 += 4) {
+        float2 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = load(s0 - r + min(i0 + k, E - 1));
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (i0 + k < E) {
+                const uint32_t h = hrow(v[k]);
+                rg[i0 + k][lane] = h;
+                V += h;
+            }
+    }
+    // q[k] holds the input row entering the window at step y = yb + k
+    float2 q[BOXI_PF];
+#pragma unroll
+    for (int k = 0; k < BOXI_PF; k++) q[k] = load(s0 + r + 1 + k);
+    int slot = 0;   // ring slot of row y - r
+    auto step = [&](int y, uint32_t hn) {
+        if (outlane) stream_store(&dst[(size_t)y * W + x], div_count((float)V, area));
+        V += hn - rg[slot][lane];
+        rg[slot][lane] = hn;
+        if (++slot == E) slot = 0;
+    };
+    int yb = s0;
+    // whole groups of BOXI_PF rows: the group's row sums are independent of
+    // each other and formed first, then the column sum steps through them
+    for (; yb + BOXI_PF <= s1; yb += BOXI_PF) {
+        uint32_t hn[BOXI_PF];
+#pragma unroll
+        for (int k = 0; k < BOXI_PF; k++) hn[k] = hrow(q[k]);   // row yb + k + r + 1
+#pragma unroll
+        for (int k = 0; k < BOXI_PF; k++) q[k] = load(yb + k + r + 1 + BOXI_PF);
+#pragma unroll
+        for (int k = 0; k < BOXI_PF; k++) step(yb + k, hn[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < BOXI_PF; k++)
+        if (yb + k < s1) step(yb + k, hrow(q[k]));
 }
 
 // ---------------------------------------------------------------- adaptive weight
@@ -1019,6 +1134,26 @@ rts_status rts_box(const float *in, float *out, int W, int H, int D, int r, void
         dim3 grid(div_up(W, 64), div_up(H, 4), D), block(64, 4);
         k_box_f32_global<<<grid, block, 0, S(stream)>>>(in, out, W, H, r);
     }
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+rts_status rts_box_int(const float *in, float *out, int W, int H, int D, int r, void *stream) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(r >= 0 && r <= BOXI_RMAX && r < W && r < H, "box radius %d must be in [0, %d] and fit the image", r,
+              BOXI_RMAX);
+    RTS_CHECK(in != out, "rts_box_int is out of place");
+    const int ntx = div_up(W, 64);
+    // segments of at least 64 rows, enough of them for about 16384 waves
+    // (two rounds of the chip at eight waves per SIMD)
+    const long cols = (long)ntx * D;
+    const int most = H / 64 > 1 ? H / 64 : 1;
+    const int nseg = (int)std::min<long>(most, std::max<long>(1, (16384 + cols - 1) / cols));
+    const int seg = div_up(H, nseg);
+    const long nwork = cols * nseg;
+    RTS_CHECK(nwork < (1L << 31) - 4, "volume too large");
+    const size_t lds = sizeof(uint32_t) * 4 * (2 * r + 1) * 64;
+    k_box_int<<<(unsigned)div_up(nwork, 4L), 256, lds, S(stream)>>>(in, out, W, H, r, ntx, nseg, seg, (int)nwork);
     RTS_LAUNCHED();
     return RTS_OK;
 }

@@ -320,6 +320,16 @@ const float kLoG5[25] = {0.0239f, 0.046f,   0.0499f, 0.046f,   0.0239f,  0.046f,
 
 bool uses(const rts_pipeline_desc &d, int cost) { return d.cost1 == cost || d.cost2 == cost; }
 
+// The cost volume holds integers no larger than 765 (colour AD, AD
+// truncated at an integer, census Hamming, rank AD): a single cost, no
+// normalisation.  The box then runs on exact integer running sums.
+bool integer_costs(const rts_pipeline_desc &d) {
+    if (d.cost2 || d.cost3) return false;
+    if (d.cost1 == RTS_COST_CENSUS || d.cost1 == RTS_COST_RANK) return true;
+    return d.cost1 == RTS_COST_AD && !d.ad_norm &&
+           (!d.ad_trunc_enable || (d.ad_trunc >= 0.0f && d.ad_trunc == (float)(int)d.ad_trunc));
+}
+
 int dev_alloc(void **p, size_t n) {
     if (!n) {
         *p = nullptr;
@@ -688,7 +698,11 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
         case RTS_AGGR_NONE:
             break;
         case RTS_AGGR_BOX:
-            if (int e = rts_box(cur, other, W, H, D, d.box_r, s)) return e;
+            if (integer_costs(d) && d.box_r <= 16) {
+                if (int e = rts_box_int(cur, other, W, H, D, d.box_r, s)) return e;
+            } else if (int e = rts_box(cur, other, W, H, D, d.box_r, s)) {
+                return e;
+            }
             std::swap(cur, other);
             break;
         case RTS_AGGR_AW:

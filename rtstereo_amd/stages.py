@@ -135,6 +135,14 @@ def box(vol, r):
     return o.download()
 
 
+def box_int(vol, r):
+    """rts_box_int: running-sum box for integer-valued volumes."""
+    D, H, W = vol.shape
+    a, o = DeviceArray.from_host(vol.astype(np.float32)), _vol(D, H, W)
+    call("rts_box_int", a.ptr, o.ptr, W, H, D, r, None)
+    return o.download()
+
+
 def prox_weight(gamma_p, r):
     p = DeviceArray(((2 * r + 1) ** 2,), np.float32)
     call("rts_prox_weight", p.ptr, float(gamma_p), r, None)

@@ -143,6 +143,32 @@ def test_box_interior_tiles(r):
     assert np.array_equal(pm(S.box(dm(c), r)), oracle.box(c, r))
 
 
+@pytest.mark.parametrize("W,H,D,seed", SIZES)
+@pytest.mark.parametrize("r", [0, 1, 4, 5, 8, 12, 16])
+def test_box_int(W, H, D, seed, r):
+    """Running-sum box on the integer-valued costs it is for (truncated AD,
+    colour AD up to 765, census Hamming): equal to the reference's row-major
+    float window sum / area, the oracle's box."""
+    if r >= min(W, H):
+        pytest.skip("radius larger than the image")
+    p = _pair(W, H, D, seed)
+    gl, gr = p.gray()
+    for c in (oracle.ad_color(p.left, p.right, D, 1, 20.0, False),
+              oracle.ad_color(p.left, p.right, D, 1, None, False),
+              oracle.hamming(oracle.census(gl, 4, 3), oracle.census(gr, 4, 3), 4, 3, D)):
+        assert np.array_equal(pm(S.box_int(dm(c), r)), oracle.box(c, r))
+
+
+def test_box_int_full_size_equals_box():
+    """At 1920x1080, D = 64 (the plain configuration's AD truncated at 20,
+    r = 4 and 5): the running-sum box equals the row-major box bit for bit."""
+    print("[case] box_int vs box 1920x1080 D=64", flush=True)
+    p = gen.make_pair(1920, 1080, 64, 20, 2.5, 77)
+    c = dm(oracle.ad_color(p.left, p.right, 64, 1, 20.0, False))
+    for r in (4, 5):
+        assert np.array_equal(S.box_int(c, r), S.box(c, r))
+
+
 @pytest.mark.parametrize("W,H,D,seed", SIZES[:3])
 def test_aw_weights(W, H, D, seed):
     p = _pair(W, H, D, seed)
