@@ -1144,10 +1144,10 @@ rts_status rts_box_int(const float *in, float *out, int W, int H, int D, int r, 
               BOXI_RMAX);
     RTS_CHECK(in != out, "rts_box_int is out of place");
     const int ntx = div_up(W, 64);
-    // segments of at least 64 rows, enough of them for about 16384 waves
+    // segments of at least 16 rows, enough of them for about 16384 waves
     // (two rounds of the chip at eight waves per SIMD)
     const long cols = (long)ntx * D;
-    const int most = H / 64 > 1 ? H / 64 : 1;
+    const int most = H / 16 > 1 ? H / 16 : 1;
     const int nseg = (int)std::min<long>(most, std::max<long>(1, (16384 + cols - 1) / cols));
     const int seg = div_up(H, nseg);
     const long nwork = cols * nseg;
