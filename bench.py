@@ -12,11 +12,19 @@ replay the frame's launch sequence as one HIP graph (captured in the
 warm-up); the per-stage split comes from a separate eager pass with HIP
 events between the stages.
 
-Prints one JSON line.  Options:
+Prints one JSON line (progress lines go to stderr).  Options:
   --gpus N      only 1 is supported (one GPU per box)
-  --steps K     timed frames (default 30)
-  --warmup W    untimed frames (default 3)
+  --steps K     timed frames (default 100)
+  --warmup W    untimed frames (default 20)
   --cpu-seconds target length of the CPU baseline sample (default 12; 0 = skip)
+  --no-aw, --no-configs, --no-batch-serve, --no-copy-probe  skip those parts
+
+Beside the headline frame the line carries the adaptive-weight FP32/FP16
+frames (`aw`), the 384x288 plain and 720p four-pass configurations
+(`other_configs`), and, each measured by a child process on the same GPU,
+the batched rank + LoG + NCC and adaptive-weight FP16 forms against
+per-pair pipelines and a 2 s serving run (`batch_rank_log_ncc`,
+`batch_aw_fp16_vs_fp32`, `serving`).
 
 cpu_baseline is the reference's seq/ stage functions on one core;
 cpu_baseline_all_cores runs the same pipeline on independent row strips, one
@@ -250,6 +258,60 @@ def config_timing(stream, frames: int = 50) -> dict:
     return res
 
 
+def child_json(module_args, timeout_s):
+    """Runs `python3 -m <module_args>` as a child process (its own HIP
+    context and environment) and returns the JSON object it prints, or an
+    error record; the bench line never fails for it."""
+    import subprocess
+    t0 = time.perf_counter()
+    try:
+        p = subprocess.run([sys.executable, "-m", *module_args], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=timeout_s, cwd=os.path.dirname(os.path.abspath(__file__)))
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout_s} s", "args": module_args}
+    if p.returncode != 0:
+        return {"error": f"exit status {p.returncode}", "stderr_tail": p.stderr[-500:], "args": module_args}
+    try:
+        out = json.loads(p.stdout.strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": "no JSON line", "args": module_args}
+    out["command"] = "python3 -m " + " ".join(module_args)
+    out["wall_s"] = time.perf_counter() - t0
+    return out
+
+
+def batch_and_serving():
+    """The batched and served forms, each measured in a child process on the
+    same GPU: 64 mixed rank + LoG + NCC pairs (640x480..2560x1440, d_max
+    64..256) against the same pairs through per-shape pipelines; 16 mixed
+    adaptive-weight FP16 pairs against the per-frame FP32 pipelines; and 2 s
+    of serving (8 cameras x 30 fps of 720p/128 census + cross, 40 AW FP16
+    bursts/s)."""
+    res = {}
+    progress("batch: 64 rank + LoG + NCC pairs")
+    b = child_json(["rtstereo_amd.batch", "--pairs", "64", "--runs", "3"], 300)
+    res["batch_rank_log_ncc"] = {k: b[k] for k in ("pairs", "pd", "batch_ms", "batch_pd_per_s", "pairs_per_s",
+                                                   "baseline_single_pipelines_ms", "speedup",
+                                                   "disparity_agreement_batched_vs_baseline", "tuned", "command",
+                                                   "error") if k in b}
+    progress("batch: 16 adaptive-weight FP16 pairs against per-frame FP32")
+    b = child_json(["rtstereo_amd.batch", "--config", "aw_fp16", "--baseline", "aw_fp32", "--pairs", "16", "--runs",
+                    "2"], 300)
+    res["batch_aw_fp16_vs_fp32"] = {k: b[k] for k in ("pairs", "pd", "batch_ms", "batch_pd_per_s", "mean_ms_per_pair",
+                                                      "baseline_single_pipelines_ms", "baseline_pd_per_s", "speedup",
+                                                      "disparity_agreement_batched_vs_baseline", "command", "error")
+                                    if k in b}
+    progress("serving: 2 s at 40 bursts/s")
+    v = child_json(["rtstereo_amd.serve", "--duration", "2", "--burst-rate", "40"], 300)
+    res["serving"] = {k: v[k] for k in ("camera", "burst", "offered", "camera_identical_to_alone",
+                                        "burst_fp16_agreement_with_fp32_alone", "command", "error") if k in v}
+    return res
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -260,6 +322,8 @@ def main() -> int:
     ap.add_argument("--no-aw", action="store_true", help="skip the adaptive-weight FP32/FP16 timing")
     ap.add_argument("--no-configs", action="store_true", help="skip the 384x288 plain and 720p 4-pass timings")
     ap.add_argument("--aw-only", action="store_true", help="print only the adaptive-weight FP32/FP16 timing")
+    ap.add_argument("--no-batch-serve", action="store_true",
+                    help="skip the batched (rank + LoG + NCC, AW FP16) and serving measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (each box has one GPU, "
@@ -381,10 +445,15 @@ def main() -> int:
         "copy_kernel_gbs": copy,
     }
     if not args.no_aw:
+        progress("adaptive weights FP32 / FP16")
         out["aw"] = aw_timing(stream, AW_ROWS if args.cpu_seconds > 0 else None)
     if not args.no_configs:
+        progress("other configurations")
         out["other_configs"] = config_timing(stream)
+    if not args.no_batch_serve:
+        out.update(batch_and_serving())
     if args.cpu_seconds > 0:
+        progress("CPU baseline")
         cb, cbn = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = cb
         out["cpu_baseline_all_cores"] = cbn
