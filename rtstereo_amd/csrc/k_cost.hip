@@ -456,9 +456,11 @@ __global__ void __launch_bounds__(256) k_cost_rank_log_ncc(const uint8_t *__rest
 
 // The same cost for a (2HR+1) x (2WR+1) NCC window by a row sweep.  Block:
 // 64 columns x a segment of NS_SEG output rows, four waves each on its own
-// disparity, DCH/4 disparity rounds.  Per row each wave stages (a, b) of the
-// columns its lanes' windows touch as 16-bit pairs in LDS and forms the
-// row's horizontal window sums of a, a^2, b, b^2, ab, a^2b, ab^2, a^2b^2;
+// disparity, DCH/4 disparity rounds.  Per row each wave stages the columns
+// its lanes' windows touch in LDS (with the view records: each column's
+// products ab, a^2b, ab^2, a^2b^2, formed once; without: the (a, b) pair)
+// and forms the row's horizontal window sums of a, a^2, b, b^2, ab, a^2b,
+// ab^2, a^2b^2 (with the records only the last four);
 // the last 2HR+1 of them stay in a register ring, so the vertical sums move
 // by one add and one subtract per row.  The window sums are the exact
 // integers of k_cost_rank_log_ncc; the quotient differs from that kernel's
@@ -536,7 +538,11 @@ __global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restric
                                                         float l2, float l3) {
     constexpr int E = 2 * HR + 1, CW = 64 + 2 * WR;
     constexpr long N = (long)E * (2 * WR + 1);
-    __shared__ uint16_t buf[4][CW];
+    // PRE: a column's four products with both views (ab, a^2 b, a b^2,
+    // a^2 b^2: all below 2^32) are formed once, where the column is
+    // staged, and the taps read them; otherwise the (a, b) pair is staged
+    __shared__ uint16_t buf[PRE ? 1 : 4][PRE ? 1 : CW];
+    __shared__ uint4 pbuf[PRE ? 4 : 1][PRE ? CW : 1];
     const float k1 = -1.4426950408889634f / l1, k2 = -1.4426950408889634f / l2, k3 = -1.4426950408889634f / l3;
     const int x0 = blockIdx.x * 64, s0 = blockIdx.y * NS_SEG, s1 = min(H, s0 + NS_SEG);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -567,13 +573,29 @@ __global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restric
                     xb = xb < 0 ? xb + W : (xb >= W ? xb - W : xb);
                     if (xa < 0 || xa >= W) xa = wrapn(xa, W);   // images narrower than the window reach
                     if (xb < 0 || xb >= W) xb = wrapn(xb, W);
-                    buf[w][c] = (uint16_t)(gl[(size_t)y * W + xa] | (gr[(size_t)y * W + xb] << 8));
+                    const unsigned a = gl[(size_t)y * W + xa], b = gr[(size_t)y * W + xb];
+                    if constexpr (PRE) {
+                        const unsigned ab = a * b;
+                        pbuf[w][c] = make_uint4(ab, a * ab, b * ab, ab * ab);
+                    } else {
+                        buf[w][c] = (uint16_t)(a | (b << 8));
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 NccSums h = {};
+                if constexpr (PRE) {
 #pragma unroll
-                for (int dx = 0; dx <= 2 * WR; dx++) {
+                    for (int dx = 0; dx <= 2 * WR; dx++) {
+                        const uint4 q = pbuf[w][lane + dx];
+                        h.sab += (int)q.x;
+                        h.saab += (int)q.y;
+                        h.sabb += (int)q.z;
+                        h.saabb += (unsigned long)q.w;
+                    }
+                }
+#pragma unroll
+                for (int dx = 0; dx <= 2 * WR && !PRE; dx++) {
                     const int v = buf[w][lane + dx];
                     const int a = v & 255, b = v >> 8, ab = a * b;
                     if constexpr (!PRE) {
