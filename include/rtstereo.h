@@ -243,6 +243,22 @@ enum { RTS_CROSS_EXACT = 0, RTS_CROSS_FAST = 1, RTS_CROSS_ARMS_LE17 = 0x100 };
 rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, const uint32_t *arms_tar, int W, int H,
                        int D, int mode, void *stream);
 
+/* rts_cross_v for arms of at most 17 followed by rts_wta (minimum, the
+ * lowest d on ties) and the left-right check, in one pass that never stores
+ * the aggregated volume: seq/CrossBasedAggregation.cpp:127-169 then
+ * seq/WTA.cpp:10-39 (L2R, and R2L from the volume's diagonal) and
+ * seq/CrossCheck.cpp:10-23.  dl, dr and mask equal rts_cross_v + rts_wta's
+ * bit for bit.  dr and mask may be null (dl only); with dr, ws holds
+ * rts_cross_v_wta_ws_bytes(W, H) bytes (8-byte aligned) of the right view's
+ * running bests.  It needs no aggregated volume (1.59 GB less device memory
+ * at 1080p/192) but takes longer than the two calls (1.76 against 1.28 ms
+ * there): for callers short of memory, not of time.  No reference
+ * counterpart as one call. */
+size_t rts_cross_v_wta_ws_bytes(int W, int H);
+rts_status rts_cross_v_wta(const float *P, const uint32_t *arms_ref, const uint32_t *arms_tar, uint8_t *dl,
+                           uint8_t *dr, uint8_t *mask, int W, int H, int D, int tol, void *ws, size_t ws_bytes,
+                           void *stream);
+
 /* One full cross-based aggregation from a raw cost volume (no prefix volume
  * in HBM): each segment sum adds its costs in column order, each output adds
  * its rows' segments in row order.  Equal to rts_hprefix + rts_cross_v in

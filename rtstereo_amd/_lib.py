@@ -84,6 +84,7 @@ _SIGS = {
     "rts_arms_unpack": [vp, vp, vp, i32, i32, vp], "rts_arms_pack": [vp, vp, vp, i32, i32, vp],
     "rts_hprefix": [vp, i32, i32, i32, vp],
     "rts_cross_v": [vp, vp, vp, vp, i32, i32, i32, i32, vp],
+    "rts_cross_v_wta": [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, vp, sz, vp],
     "rts_cross_fused": [vp, vp, vp, vp, i32, i32, i32, vp],
     "rts_wta": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],
     "rts_wta_noalloc": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],
@@ -111,10 +112,11 @@ _SIGS = {
     "rts_png_free": [vp],
     "rts_png_write": [cp, vp, i32, i32, i32],
     "rts_disp_scale": [vp, i32, i32, f32],
+    "rts_cross_v_wta_ws_bytes": [i32, i32],
 }
 _RESTYPES = {"rts_last_error": cp, "rts_version": cp, "rts_pipeline_defaults": None, "rts_png_free": None,
              "rts_pipeline_bytes": sz, "rts_pipeline_volume": vp,
-             "rts_aw_workspace_bytes": sz, "rts_cost_rank_log_ncc_ws_bytes": sz}
+             "rts_aw_workspace_bytes": sz, "rts_cost_rank_log_ncc_ws_bytes": sz, "rts_cross_v_wta_ws_bytes": sz}
 
 for _name, _args in _SIGS.items():
     if not hasattr(lib, _name):

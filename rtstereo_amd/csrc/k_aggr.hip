@@ -1061,6 +1061,198 @@ __global__ void __launch_bounds__(64 * NW) k_cross_sweep(const float *__restrict
     }
 }
 
+// The vertical sweep for arms of at most CV_LMAX fused with the disparity
+// selection (seq/CrossBasedAggregation.cpp:127-169, then seq/WTA.cpp:10-39
+// for both views), so that the aggregated volume is never stored.  A block
+// of FW_NW waves owns 64 columns x FW_SEG rows of the frame and walks every
+// d.  Per d it forms the segment sums of the rows its outputs' supports can
+// reach (s0-17 .. s0+FW_SEG+16, all live at once in LDS), then each wave sums
+// its output rows s0+w+8i exactly as k_cross_sweep does, and compares the
+// value instead of storing it:
+//  - left view: a running best (value, d) per output in registers, replaced
+//    only by a strictly smaller value (d = 0 first: the lowest d wins ties);
+//  - right view: the output at column x and disparity d is a candidate for
+//    right pixel (x-d) mod W.  Lane i keeps the running best of right pixel
+//    x0+i-d; before each next d the state moves up one lane (DPP wave
+//    shift), lane 0 starts empty, and the state leaving lane 63, which this
+//    block sees no more, is merged into the frame's right-view buffer with a
+//    64-bit atomic min of (value bits << 32 | d): non-negative floats order
+//    as their bits, and the lowest d wins ties.  After the last d every
+//    lane's state is merged.
+// The next step's rows are loaded during the current step's sums.  The
+// right-view buffer starts at all ones (k_fill_u64).
+constexpr int FW_SEG = 64, FW_NW = 8;
+constexpr int FW_ROWS = FW_SEG + 2 * CV_LMAX;       // rows live at once
+constexpr int FW_RING = FW_ROWS + 8;                // + rows a short support's grouped reads pass
+constexpr int FW_KP = (FW_ROWS + FW_NW - 1) / FW_NW;   // rows each wave forms per d
+constexpr int FW_KO = FW_SEG / FW_NW;                  // output rows per wave
+
+__device__ __forceinline__ unsigned long long fw_pack(float v, int d) {
+    return (unsigned long long)__float_as_uint(v) << 32 | (unsigned)d;
+}
+
+template <bool WANT_R>
+__global__ void __launch_bounds__(64 * FW_NW) k_cross_sweep_wta(const float *__restrict__ P,
+                                                                const uint32_t *__restrict__ aref,
+                                                                const uint32_t *__restrict__ atar,
+                                                                uint8_t *__restrict__ dl,
+                                                                unsigned long long *__restrict__ r2l, int W, int H,
+                                                                int D, int nx, int nseg) {
+    __shared__ float2 ring[FW_RING][CV_TX];
+    // segments fastest: an XCD's blocks hold whole column strips, whose
+    // neighbouring segments read each other's halo rows (at about the same
+    // d) from its L2
+    const int wk = xcd_order(blockIdx.x, nx * nseg);
+    const int xt = wk / nseg, sg = wk - xt * nseg;
+    const int x0 = xt * CV_TX, s0 = sg * FW_SEG, s1 = min(H, s0 + FW_SEG);
+    const int rlo = max(0, s0 - CV_LMAX), rhi = min(H, s1 + CV_LMAX);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int x = x0 + lane;
+    const bool colok = x < W;
+    const int xc = min(x, W - 1);
+    const int cs = x0 - CV_LMAX - 1;
+    // prefix columns cs .. cs+98 in two registers per lane (k_cross_sweep)
+    const int c0 = cs + lane, c1 = cs + lane + (lane < CV_COLS - 64 ? 64 : 0);
+    const uint32_t oc0 = (uint32_t)min(max(c0, 0), W - 1) * 4u, oc1 = (uint32_t)min(max(c1, 0), W - 1) * 4u;
+    const bool interior = cs >= 0 && cs + CV_COLS <= W;
+    const uint32_t oxc = (uint32_t)xc * 4u;
+    const bool hi_l = lane >= 47, hi_r = lane >= 29;
+    const uint64_t pb64 = (uint64_t)W * (uint64_t)H * 4u;
+    const int pbytes = (int)(pb64 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)pb64);
+    const size_t pl = plane(W, H);
+    const auto rsR = __builtin_amdgcn_make_buffer_rsrc((void *)aref, 0, pbytes, BUF_DWORD3);
+    const auto rsT = __builtin_amdgcn_make_buffer_rsrc((void *)atar, 0, pbytes, BUF_DWORD3);
+    auto row_off = [&](int k) { return (uint32_t)min(rlo + w + FW_NW * k, rhi - 1) * (uint32_t)W * 4u; };
+    // the reference view's arms do not depend on d
+    uint32_t ar[FW_KP];
+#pragma unroll
+    for (int k = 0; k < FW_KP; k++) ar[k] = __builtin_amdgcn_raw_buffer_load_b32(rsR, oxc, row_off(k), 0);
+    float p0[FW_KP], p1[FW_KP];
+    uint32_t at[FW_KP];
+    auto load_d = [&](int d) {
+        const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void *)(P + (size_t)d * pl), 0, pbytes, BUF_DWORD3);
+        const uint32_t otx = (uint32_t)wrap1(xc - d, W) * 4u;
+#pragma unroll
+        for (int k = 0; k < FW_KP; k++) {
+            const uint32_t ro = row_off(k);
+            p0[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsP, oc0, ro, 0));
+            p1[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rsP, oc1, ro, 0));
+            at[k] = __builtin_amdgcn_raw_buffer_load_b32(rsT, otx, ro, 0);
+        }
+    };
+    // segment sums of this d's rows into the ring (the entry format and the
+    // arithmetic of k_cross_sweep)
+    auto produce = [&](auto interior_tag) {
+        constexpr bool INTERIOR = decltype(interior_tag)::value;
+#pragma unroll
+        for (int k = 0; k < FW_KP; k++) {
+            const uint32_t a = ar[k], c = at[k];
+            const uint32_t l = min(a & 0xFFu, c & 0xFFu), rr = min((a >> 8) & 0xFFu, (c >> 8) & 0xFFu);
+            const uint32_t t = min((a >> 16) & 0xFFu, (c >> 16) & 0xFFu), b = min(a >> 24, c >> 24);
+            const int jl4 = INTERIOR ? 4 * (lane + CV_LMAX) - 4 * (int)l : 4 * (max(xc - (int)l - 1, 0) - cs);
+            const int jr4 = INTERIOR ? 4 * (int)rr + 4 * (lane + CV_LMAX + 1) : 4 * (min(xc + (int)rr, W - 1) - cs);
+            const int bl = __builtin_amdgcn_ds_bpermute(jl4, __float_as_int(p0[k]));
+            const int bh = __builtin_amdgcn_ds_bpermute(jl4, __float_as_int(p1[k]));
+            const int el = __builtin_amdgcn_ds_bpermute(jr4, __float_as_int(p0[k]));
+            const int eh = __builtin_amdgcn_ds_bpermute(jr4, __float_as_int(p1[k]));
+            const bool uh_l = INTERIOR ? hi_l : jl4 >= 256, uh_r = INTERIOR ? hi_r : jr4 >= 256;
+            const float sv = __int_as_float(uh_r ? eh : el) - __int_as_float(uh_l ? bh : bl);
+            const int row = rlo + w + FW_NW * k;
+            if (row < rhi) ring[row - rlo][lane] = make_float2(sv, __uint_as_float((l + rr + 1) | t << 16 | b << 24));
+        }
+    };
+    // the aggregated value of the output whose own row sits at ring position py
+    auto out_row = [&](int py) -> float {
+        const uint32_t wd = __float_as_uint(ring[py][lane].y);
+        const int t = (wd >> 16) & 0xFF, b = wd >> 24;
+        const int n = t + b + 1, lead = n & 3, q = n >> 2;
+        const float2 *s = &ring[py - t][lane];
+        float sum = 0.0f;
+        uint32_t cw = 0;
+        if (lead) {
+            const float2 a0 = s[0], a1 = s[CV_TX], a2 = s[2 * CV_TX];
+            const bool u1 = lead > 1, u2 = lead > 2;
+            sum = ((sum + a0.x) + (u1 ? a1.x : 0.0f)) + (u2 ? a2.x : 0.0f);
+            cw = __float_as_uint(a0.y) + (u1 ? __float_as_uint(a1.y) : 0u) + (u2 ? __float_as_uint(a2.y) : 0u);
+        }
+        s += lead * CV_TX;
+#pragma unroll 1
+        for (int gi = 0; gi < q; gi++) {
+            const float2 *gp = s + 4 * gi * CV_TX;
+            const float2 e0 = gp[0], e1 = gp[CV_TX], e2 = gp[2 * CV_TX], e3 = gp[3 * CV_TX];
+            sum = (((sum + e0.x) + e1.x) + e2.x) + e3.x;
+            cw += __float_as_uint(e0.y) + __float_as_uint(e1.y) + __float_as_uint(e2.y) + __float_as_uint(e3.y);
+        }
+        return div_count(sum, cw & 0xFFFFu);
+    };
+    const float INF = __uint_as_float(0x7F800000u);
+    float bL[FW_KO], bR[FW_KO];
+    int dL[FW_KO], dR[FW_KO];
+#pragma unroll
+    for (int i = 0; i < FW_KO; i++) {
+        bL[i] = bR[i] = INF;
+        dL[i] = dR[i] = 0;
+    }
+    load_d(0);
+    for (int d = 0; d < D; d++) {
+        if (interior)
+            produce(std::true_type{});
+        else
+            produce(std::false_type{});
+        __syncthreads();
+        if (d + 1 < D) load_d(d + 1);
+#pragma unroll
+        for (int i = 0; i < FW_KO; i++) {
+            const int y = s0 + w + FW_NW * i;
+            if (y >= s1) break;
+            const float v = colok ? out_row(y - rlo) : INF;
+            if (d == 0 || v < bL[i]) {
+                bL[i] = v;
+                dL[i] = d;
+            }
+            if (WANT_R && v < bR[i]) {
+                bR[i] = v;
+                dR[i] = d;
+            }
+        }
+        if (WANT_R) {
+            // the state of right pixel x0+63-d leaves the block; the others
+            // move up one lane, and lane 0 starts right pixel x0-d-1
+#pragma unroll
+            for (int i = 0; i < FW_KO; i++) {
+                const int y = s0 + w + FW_NW * i;
+                if (y >= s1) break;
+                if (lane == 63 && bR[i] != INF)
+                    atomicMin(&r2l[(size_t)y * W + wrapn(x0 + 63 - d, W)], fw_pack(bR[i], dR[i]));
+                bR[i] = __int_as_float(__builtin_amdgcn_update_dpp((int)0x7F800000, __float_as_int(bR[i]), 0x138, 0xf,
+                                                                   0xf, false));   // wave_shr:1
+                dR[i] = __builtin_amdgcn_update_dpp(0, dR[i], 0x138, 0xf, 0xf, false);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < FW_KO; i++) {
+        const int y = s0 + w + FW_NW * i;
+        if (y >= s1) break;
+        if (colok) dl[(size_t)y * W + x] = (uint8_t)dL[i];
+        // after the last shift lane i holds right pixel x0+i-D
+        if (WANT_R && bR[i] != INF) atomicMin(&r2l[(size_t)y * W + wrapn(x0 + lane - D, W)], fw_pack(bR[i], dR[i]));
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long *__restrict__ p, unsigned long long v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// D_R from the merged right-view bests: the low 32 bits hold d.
+__global__ void k_r2l_finish(const unsigned long long *__restrict__ r2l, uint8_t *__restrict__ dr, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dr[i] = (uint8_t)(r2l[i] & 0xFFu);
+}
+
 }  // namespace rts
 
 using namespace rts;
@@ -1241,6 +1433,40 @@ rts_status rts_cross_v(const float *P, float *out, const uint32_t *arms_ref, con
         k_cross_strip<<<nx * nseg * D, 256, 0, S(stream)>>>(P, out, arms_ref, arms_tar, W, H, nx, nseg, D);
     }
     RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+size_t rts_cross_v_wta_ws_bytes(int W, int H) {
+    return W < 1 || H < 1 ? 0 : (size_t)W * (size_t)H * sizeof(unsigned long long);
+}
+
+rts_status rts_cross_v_wta(const float *P, const uint32_t *arms_ref, const uint32_t *arms_tar, uint8_t *dl,
+                           uint8_t *dr, uint8_t *mask, int W, int H, int D, int tol, void *ws, size_t ws_bytes,
+                           void *stream) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(dl != nullptr, "rts_cross_v_wta needs dl");
+    RTS_CHECK(mask == nullptr || dr != nullptr, "the left-right mask needs dr");
+    const size_t need = rts_cross_v_wta_ws_bytes(W, H);
+    RTS_CHECK(dr == nullptr || (ws != nullptr && ws_bytes >= need && ((uintptr_t)ws & 7) == 0),
+              "right-view workspace of %zu bytes (8-byte aligned) needed, %zu given", need, ws_bytes);
+    const int nx = div_up(W, CV_TX), nseg = div_up(H, FW_SEG);
+    hipStream_t s = S(stream);
+    const size_t n = (size_t)W * H;
+    auto *r2l = static_cast<unsigned long long *>(ws);
+    if (dr) {
+        k_fill_u64<<<(unsigned)div_up((long)n, 256L), 256, 0, s>>>(r2l, ~0ull, n);
+        RTS_LAUNCHED();
+        k_cross_sweep_wta<true><<<nx * nseg, 64 * FW_NW, 0, s>>>(P, arms_ref, arms_tar, dl, r2l, W, H, D, nx, nseg);
+        RTS_LAUNCHED();
+        k_r2l_finish<<<(unsigned)div_up((long)n, 256L), 256, 0, s>>>(r2l, dr, n);
+        RTS_LAUNCHED();
+        if (mask)
+            if (int e = rts_cross_check(dl, dr, mask, W, H, tol, stream)) return e;
+    } else {
+        k_cross_sweep_wta<false><<<nx * nseg, 64 * FW_NW, 0, s>>>(P, arms_ref, arms_tar, dl, nullptr, W, H, D, nx,
+                                                                  nseg);
+        RTS_LAUNCHED();
+    }
     return RTS_OK;
 }
 

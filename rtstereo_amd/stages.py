@@ -206,6 +206,22 @@ def cross_v(P, arms_ref, arms_tar, mode=RTS_CROSS_EXACT):
     return o.download()
 
 
+def cross_v_wta(P, arms_ref, arms_tar, lr=True, tol=0):
+    """rts_cross_v_wta: the arms <= 17 sweep fused with the WTA; returns
+    (D_L, D_R, mask) (D_R and mask None unless lr)."""
+    D, H, W = P.shape
+    a = DeviceArray.from_host(P.astype(np.float32))
+    ar, at = DeviceArray.from_host(arms_ref.astype(np.uint32)), DeviceArray.from_host(arms_tar.astype(np.uint32))
+    dl = DeviceArray((H, W), np.uint8)
+    dr = DeviceArray((H, W), np.uint8) if lr else None
+    m = DeviceArray((H, W), np.uint8) if lr else None
+    nws = int(lib.rts_cross_v_wta_ws_bytes(W, H))
+    ws = DeviceArray((nws,), np.uint8) if lr else None
+    call("rts_cross_v_wta", a.ptr, ar.ptr, at.ptr, dl.ptr, dr.ptr if lr else None, m.ptr if lr else None, W, H, D,
+         tol, ws.ptr if lr else None, nws if lr else 0, None)
+    return dl.download(), (dr.download() if lr else None), (m.download() if lr else None)
+
+
 def cross_fused(cost, arms_ref, arms_tar):
     D, H, W = cost.shape
     a, o = DeviceArray.from_host(cost.astype(np.float32)), _vol(D, H, W)
