@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(256) k_cost_rln_sweep(const uint8_t *__restric
             for (int k = 0; k < E; k++) {
                 const int i = base + k;
                 if (i >= nr) break;
-                const int y = wrapn(s0 - HR + i, H);
+                const int y = wrap1(s0 - HR + i, H);   // 2HR+1 <= H: at most one period out
                 // Each wave works on its own LDS row: a wave-level fence and
                 // barrier order its previous reads before these writes, and
                 // these writes before the reads below.
