@@ -189,6 +189,15 @@ def aw_timing(stream, rows=None) -> dict:
                                   "valu_instr_per_tap": AW_VALU_PER_TAP[name], "valu_bound_ms": valu_ms,
                                   "frac_of_valu_bound": valu_ms / aggr}}
         if rows is not None:
+            # the accuracy run takes the host Lab images the FP64 oracle
+            # uses, so that the comparison measures the aggregation alone
+            # (the device's FP32 Lab conversion may differ by one level)
+            hl = p.lab()
+            la, lb = DeviceArray.from_host(hl[0]), DeviceArray.from_host(hl[1])
+            pipe.run(a, b, la, lb, stream=stream.handle)
+            stream.sync()
+            la.free()
+            lb.free()
             vol = pipe.volume()
             res[name]["rows"] = np.moveaxis(vol[:, rows, :], 0, -1).astype(np.float64)
             del vol
