@@ -27,9 +27,12 @@
 
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 namespace rts {
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int AW_RMAX = 16;               // largest window radius
 constexpr int AW_TX = 64;                 // weight rows are padded to a multiple of this
@@ -320,6 +323,11 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
                     tv[g] = tw[g];
                     lv[g] = lw[g];
                 }
+                // The target row's last 16-byte word is read whole even where
+                // only its first words are taps: the lanes' rows lie PG words
+                // apart, and 4-byte reads at that stride share banks, where
+                // 16-byte reads do not.
+                if constexpr (NT % 4 != 0) asm volatile("" ::"v"(__builtin_bit_cast(u32x4, tv[PG / 4 - 1])));
                 const uint32_t *t32 = reinterpret_cast<const uint32_t *>(tv);
                 const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lv);
                 h2 p[NT];
@@ -369,6 +377,8 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
                         tv[g] = tw[g];
                         lv[g] = lw[g];
                     }
+                    // the last target word read whole (as in the FP16 form)
+                    if constexpr (NT % 4 != 0) asm volatile("" ::"v"(__builtin_bit_cast(u32x4, tv[PG / 4 - 1])));
                     const float *tf = reinterpret_cast<const float *>(tv);
                     const float *lf = reinterpret_cast<const float *>(lv);
                     float sn = num[ny][k], sd = den[ny][k];
@@ -417,9 +427,275 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
     }
 }
 
+// FP16 aggregation with its staging by direct loads into LDS (global_load_lds:
+// no registers, no LDS store instructions), double-buffered so that window
+// row yy+1 arrives while row yy is summed.  Same tile (64 pixels x 2 rows x
+// 64 disparities, lane = disparity, 8 waves) and the same arithmetic as
+// k_aw_band<true, R>; the cost rows stay FP32 in LDS (an odd row pitch, so
+// that the per-lane row reads meet no bank conflicts) and each lane packs its
+// register window of half pairs with v_cvt_pk_f16_f32, which rounds as the
+// staged conversion did.  The two buffers are separate LDS arrays: the
+// compiler then knows that the loads into one do not feed the reads of the
+// other and waits only as the code asks.
+template <int R, int NY>
+struct AwDma {
+    static constexpr int NWAVE = 8, NTHR = 64 * NWAVE;
+    static constexpr int TXP = 64, QW = TXP + 64;
+    static constexpr int NT = R + 1, PG = (NT + 3) & ~3;
+    // FP32 cost columns per disparity row: the floats a lane's pairs reach
+    // (up to 16*3 + 1 + 2*(AW_NX+R) - 1), at an odd pitch
+    static constexpr int PCF = (49 + 2 * (AW_NX + R)) | 1;
+    static constexpr int C_INS = (64 * PCF + 63) / 64;               // 4-byte loads of 64 lanes
+    static constexpr int T_QUADS = NY * QW * PG / 4, L_QUADS = NY * TXP * PG / 4;
+    static constexpr int T_INS = (T_QUADS + 63) / 64, L_INS = (L_QUADS + 63) / 64;   // 16-byte loads
+    static constexpr int C_PW = (C_INS + NWAVE - 1) / NWAVE, T_PW = (T_INS + NWAVE - 1) / NWAVE,
+                         L_PW = (L_INS + NWAVE - 1) / NWAVE;
+    static constexpr int NLOAD = C_PW + T_PW + L_PW;                   // loads per wave per window row
+    static constexpr int T_OFF = C_INS * 64, L_OFF = T_OFF + T_INS * 256;
+    static constexpr int WORDS = L_OFF + L_INS * 256;
+    static_assert(WORDS >= 64 * 65, "the output stage reuses one buffer");
+    static_assert(2 * WORDS * 4 <= 160 * 1024, "two buffers must fit the LDS");
+    static_assert(NLOAD < 64, "vmcnt counts at most 63");
+};
+
+// Direct loads into LDS (64 lanes: lane i's 4 or 16 bytes from base + its
+// byte offset land at LDS byte address lds + 4i or 16i), written out so that
+// the compiler neither tracks nor waits for them: the kernel issues its own
+// vmcnt waits.  M0 holds the LDS address; nothing else in these kernels
+// uses M0.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ uint32_t lds_addr(const uint32_t *p) { return (uint32_t)(size_t)(const lds_u32 *)p; }
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void dma_b32(const void *base, uint32_t off, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dword %0, %1" ::"v"(off), "s"(base), "s"(lds) : "memory", "m0");
+}
+__device__ __forceinline__ void dma_b128(const void *base, uint32_t off, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %2\n\tglobal_load_lds_dwordx4 %0, %1" ::"v"(off), "s"(base), "s"(lds)
+                 : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+    // s_waitcnt with vmcnt N (bits 3:0 and 15:14), expcnt and lgkmcnt at
+    // their maxima (no wait on them)
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+template <int R, int NY>
+__global__ void __launch_bounds__(512) k_aw_band_dma(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb,
+                                                     const float *__restrict__ vol, float *__restrict__ out, int W,
+                                                     int H, int D, int Wp, int y0, int rows, int r, int nxb, int nyb,
+                                                     int ndb) {
+    using T = AwDma<R, NY>;
+    constexpr int NT = T::NT, PG = T::PG, TXP = T::TXP, QW = T::QW, PCF = T::PCF;
+    __shared__ __attribute__((aligned(16))) uint32_t bufA[T::WORDS];
+    __shared__ __attribute__((aligned(16))) uint32_t bufB[T::WORDS];
+
+    const int nblk = nxb * nyb * ndb;
+    const int lin = xcd_order(blockIdx.x, nblk);
+    const int xb = lin % nxb, rest = lin / nxb;
+    const int db = rest % ndb, yb = rest / ndb;
+    const int x0 = xb * TXP, d0 = db * 64, ry0 = yb * NY, yt = y0 + ry0;
+    const int tid = threadIdx.x, dl = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nrows = min(NY, rows - ry0);
+    const int e = 2 * r + 1;
+    const size_t pl = plane(W, H);
+    const int q_lo = x0 - d0 - 64;   // first target centre staged
+
+    auto row_valid = [&](int yy, int ny) {
+        const int dy = yy - (yt + ny);
+        return ny < nrows && dy >= -r && dy <= r;
+    };
+    // workspace row of tile row ny for window row yy, clamped into the
+    // tile row's own rows when yy lies outside its window (data then unused)
+    auto wrow_index = [&](int yy, int ny) {
+        const int nyc = min(ny, nrows - 1);
+        const int dy = min(max(yy - (yt + nyc), -r), r);
+        return (size_t)(ry0 + nyc) * e + (dy + r);
+    };
+
+    // Per-lane byte offsets of the loads, fixed for the block.
+    uint32_t offc[T::C_PW], offt[T::T_PW], offl[T::L_PW];
+    uint32_t nyt = 0, nyl = 0;   // bit k: the k-th weight load belongs to tile row 1
+#pragma unroll
+    for (int k = 0; k < T::C_PW; k++) {
+        const int ins = min(wave + T::NWAVE * k, T::C_INS - 1);
+        const int w = ins * 64 + dl;
+        const int dd = min(w / PCF, 63), j = w - (w / PCF) * PCF;
+        const int d = min(d0 + dd, D - 1);
+        offc[k] = 4u * ((uint32_t)((size_t)(d - d0) * pl) + (uint32_t)wrap_any(x0 - R + j, W));
+    }
+#pragma unroll
+    for (int k = 0; k < T::T_PW; k++) {
+        const int ins = min(wave + T::NWAVE * k, T::T_INS - 1);
+        const int u = min(ins * 64 + dl, T::T_QUADS - 1);
+        const int ny = u / (QW * PG / 4), v4 = u - ny * (QW * PG / 4);
+        const int q = v4 / (PG / 4), g = v4 - q * (PG / 4);
+        offt[k] = 4u * ((uint32_t)wrap_any(q_lo + q, W) * PG + 4 * g);
+        nyt |= (uint32_t)ny << k;
+    }
+#pragma unroll
+    for (int k = 0; k < T::L_PW; k++) {
+        const int ins = min(wave + T::NWAVE * k, T::L_INS - 1);
+        const int u = min(ins * 64 + dl, T::L_QUADS - 1);
+        const int ny = u / (TXP * PG / 4), v4 = u - ny * (TXP * PG / 4);
+        const int px = v4 / (PG / 4), g = v4 - px * (PG / 4);
+        offl[k] = 4u * ((uint32_t)(x0 + px) * PG + 4 * g);
+        nyl |= (uint32_t)ny << k;
+    }
+
+    // Window row yy into buffer BUF: the FP32 cost rows of the block's 64
+    // disparities (columns x0-R .., wrapped), the target weight rows of the
+    // QW centres from x0-d0-64 and the reference rows of the tile's pixels.
+    // The weight rows of tile row 1 start (row 1's index - row 0's) rows of
+    // Wp*PG words after row 0's; that distance is added to the lanes whose
+    // load belongs to tile row 1 (uniform per window row).
+    auto issue = [&](auto buf_tag, int yy) {
+        const uint32_t *buf = decltype(buf_tag)::value == 0 ? bufA : bufB;
+        const float *crow = vol + (size_t)d0 * pl + (size_t)wrap_any(yy, H) * W;
+#pragma unroll
+        for (int k = 0; k < T::C_PW; k++) {
+            const int ins = min(wave + T::NWAVE * k, T::C_INS - 1);
+            dma_b32(crow, offc[k], lds_addr(buf + ins * 64));
+        }
+        const size_t i0 = wrow_index(yy, 0), i1 = wrow_index(yy, 1);
+        const uint32_t step1 = (uint32_t)((i1 - i0) * (size_t)Wp * PG * 4);
+        const uint32_t *tb0 = wtb + i0 * Wp * PG, *lb0 = wlb + i0 * Wp * PG;
+#pragma unroll
+        for (int k = 0; k < T::T_PW; k++) {
+            const int ins = min(wave + T::NWAVE * k, T::T_INS - 1);
+            dma_b128(tb0, offt[k] + ((nyt >> k) & 1 ? step1 : 0u), lds_addr(buf + T::T_OFF + ins * 256));
+        }
+#pragma unroll
+        for (int k = 0; k < T::L_PW; k++) {
+            const int ins = min(wave + T::NWAVE * k, T::L_INS - 1);
+            dma_b128(lb0, offl[k] + ((nyl >> k) & 1 ? step1 : 0u), lds_addr(buf + T::L_OFF + ins * 256));
+        }
+    };
+
+    float num[NY][AW_NX], den[NY][AW_NX];
+#pragma unroll
+    for (int a = 0; a < NY; a++)
+#pragma unroll
+        for (int k = 0; k < AW_NX; k++) num[a][k] = den[a][k] = 0.0f;
+
+    // The sums of window row yy from buffer BUF (the arithmetic of
+    // k_aw_band<true, R>).  Tap pair j of lane column k covers tile
+    // columns px+2j, px+2j+1 with px = 16*(wave>>1) + parity + 2k: floats
+    // 16*(wave>>1) + parity + 2(k+j) and the next of the lane's staged row.
+    auto compute = [&](auto buf_tag, int yy) {
+        const uint32_t *buf = decltype(buf_tag)::value == 0 ? bufA : bufB;
+        const float *cr = reinterpret_cast<const float *>(buf) + dl * PCF + 16 * (wave >> 1) + (wave & 1);
+        h2 cw[AW_NX + R];
+#pragma unroll
+        for (int i = 0; i < AW_NX + R; i++) {
+            h2 v;
+            v.x = (_Float16)cr[2 * i];
+            v.y = (_Float16)cr[2 * i + 1];
+            cw[i] = v;
+        }
+        const h2 one = {(_Float16)1.0f, (_Float16)1.0f};
+        const uint32_t *Lt = buf + T::L_OFF;
+#pragma unroll
+        for (int ny = 0; ny < NY; ny++) {
+            if (!row_valid(yy, ny)) continue;
+#pragma unroll
+            for (int k = 0; k < AW_NX; k++) {
+                const int px = aw_px<true>(wave, k);
+                const uint4 *tw = reinterpret_cast<const uint4 *>(buf + opaque(T::T_OFF + (ny * QW + px - dl + 64) * PG));
+                const uint4 *lw = reinterpret_cast<const uint4 *>(Lt + (ny * TXP + px) * PG);
+                uint4 tv[PG / 4], lv[PG / 4];
+#pragma unroll
+                for (int g = 0; g < PG / 4; g++) {
+                    tv[g] = tw[g];
+                    lv[g] = lw[g];
+                }
+                // The target row's last 16-byte word is read whole even where
+                // only its first words are taps: the lanes' rows lie PG words
+                // apart, and 4-byte reads at that stride share banks, where
+                // 16-byte reads do not.
+                if constexpr (NT % 4 != 0) asm volatile("" ::"v"(__builtin_bit_cast(u32x4, tv[PG / 4 - 1])));
+                const uint32_t *t32 = reinterpret_cast<const uint32_t *>(tv);
+                const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lv);
+                h2 p[NT];
+#pragma unroll
+                for (int j = 0; j < NT; j++) p[j] = __builtin_bit_cast(h2, l32[j]) * __builtin_bit_cast(h2, t32[j]);
+                float sn = num[ny][k], sd = den[ny][k];
+#pragma unroll
+                for (int j = 0; j < NT; j++) {
+                    sn = __builtin_amdgcn_fdot2(p[j], cw[k + j], sn, false);
+                    sd = __builtin_amdgcn_fdot2(p[j], one, sd, false);
+                }
+                num[ny][k] = sn;
+                den[ny][k] = sd;
+            }
+        }
+    };
+
+    using A = std::integral_constant<int, 0>;
+    using B = std::integral_constant<int, 1>;
+    const int ylast = yt + nrows - 1 + r;
+    // Each step: every wave is done reading the buffer about to be refilled
+    // (its reads have returned: lgkmcnt 0, then the barrier); the next row's
+    // loads go out; this wave's loads of the current row are waited for
+    // (vmcnt = the loads just issued) and the barrier makes every wave's part
+    // of the buffer complete.
+    auto step = [&](auto cur, auto nxt, int yy) {
+        __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        if (yy < ylast) {
+            issue(nxt, yy + 1);
+            wait_vmcnt<T::NLOAD>();
+        } else {
+            wait_vmcnt<0>();
+        }
+        __builtin_amdgcn_s_barrier();
+        compute(cur, yy);
+    };
+    issue(A{}, yt - r);
+    for (int yy = yt - r; yy <= ylast; yy += 2) {
+        step(A{}, B{}, yy);
+        if (yy + 1 <= ylast) step(B{}, A{}, yy + 1);
+    }
+
+    // Write through LDS so that each disparity row of 64 pixels is stored
+    // contiguously.
+    float *tile = reinterpret_cast<float *>(bufA);
+#pragma unroll
+    for (int ny = 0; ny < NY; ny++) {
+        if (ny >= nrows) break;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < AW_NX; k++) tile[dl * 65 + aw_px<true>(wave, k)] = num[ny][k] / den[ny][k];
+        __syncthreads();
+        const int y = yt + ny;
+        for (int i = tid; i < 64 * TXP; i += T::NTHR) {
+            const int dd = i / TXP, px = i % TXP;
+            const int d = d0 + dd, x = x0 + px;
+            if (d < D && x < W) stream_store(&out[(size_t)d * pl + (size_t)y * W + x], tile[dd * 65 + px]);
+        }
+    }
+}
+
 template <bool HALF, int R>
 int aw_band_launch(const void *wl, const void *wt, const float *vol, float *out, int W, int H, int D, int Wp, int y0,
                    int rows, int r, hipStream_t s) {
+    // FP16: the direct-load kernel, whose per-lane cost offsets span 64
+    // planes in 32 bits
+    if (HALF && 64.0 * (double)W * (double)H < 4294967296.0) {
+        // one tile row per block: two blocks (two double buffers) share a CU
+        constexpr int NY = 1;
+        using TD = AwDma<R, NY>;
+        const int nxb = div_up(W, TD::TXP), nyb = div_up(rows, NY), ndb = div_up(D, 64);
+        k_aw_band_dma<R, NY><<<nxb * nyb * ndb, TD::NTHR, 0, s>>>(static_cast<const uint32_t *>(wl),
+                                                                   static_cast<const uint32_t *>(wt), vol, out, W, H,
+                                                                   D, Wp, y0, rows, r, nxb, nyb, ndb);
+        RTS_LAUNCHED();
+        return RTS_OK;
+    }
     using T = AwTile<HALF, R>;
     constexpr size_t lds_bytes = sizeof(uint32_t) * T::WORDS;
     static bool attr_set = false;   // the attribute is per function; setting it twice is harmless
