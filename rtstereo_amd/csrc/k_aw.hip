@@ -33,6 +33,8 @@ namespace rts {
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int AW_RMAX = 16;               // largest window radius
 constexpr int AW_TX = 64;                 // weight rows are padded to a multiple of this
@@ -452,8 +454,9 @@ struct AwDma {
                          L_PW = (L_INS + NWAVE - 1) / NWAVE;
     static constexpr int NLOAD = C_PW + T_PW + L_PW;                   // loads per wave per window row
     static constexpr int T_OFF = C_INS * 64, L_OFF = T_OFF + T_INS * 256;
-    static constexpr int WORDS = L_OFF + L_INS * 256;
-    static_assert(WORDS >= 64 * 65, "the output stage reuses one buffer");
+    // (each buffer also holds, at the end, the 64 x 129 denominator matrix
+    // or the 64 x 65 output stage)
+    static constexpr int WORDS = L_OFF + L_INS * 256 > 64 * 129 ? L_OFF + L_INS * 256 : 64 * 129;
     static_assert(2 * WORDS * 4 <= 160 * 1024, "two buffers must fit the LDS");
     static_assert(NLOAD < 64, "vmcnt counts at most 63");
 };
@@ -483,12 +486,14 @@ __device__ __forceinline__ void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
 }
 
+// (four waves per SIMD: two blocks per CU, so at most 128 registers)
 template <int R, int NY>
-__global__ void __launch_bounds__(512) k_aw_band_dma(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_aw_band_dma(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb,
                                                      const float *__restrict__ vol, float *__restrict__ out, int W,
                                                      int H, int D, int Wp, int y0, int rows, int r, int nxb, int nyb,
                                                      int ndb) {
     using T = AwDma<R, NY>;
+    static_assert(NY == 1, "the matrix-core denominators cover one tile row");
     constexpr int NT = T::NT, PG = T::PG, TXP = T::TXP, QW = T::QW, PCF = T::PCF;
     __shared__ __attribute__((aligned(16))) uint32_t bufA[T::WORDS];
     __shared__ __attribute__((aligned(16))) uint32_t bufB[T::WORDS];
@@ -505,10 +510,6 @@ __global__ void __launch_bounds__(512) k_aw_band_dma(const uint32_t *__restrict_
     const size_t pl = plane(W, H);
     const int q_lo = x0 - d0 - 64;   // first target centre staged
 
-    auto row_valid = [&](int yy, int ny) {
-        const int dy = yy - (yt + ny);
-        return ny < nrows && dy >= -r && dy <= r;
-    };
     // workspace row of tile row ny for window row yy, clamped into the
     // tile row's own rows when yy lies outside its window (data then unused)
     auto wrow_index = [&](int yy, int ny) {
@@ -576,17 +577,29 @@ __global__ void __launch_bounds__(512) k_aw_band_dma(const uint32_t *__restrict_
         }
     };
 
-    float num[NY][AW_NX], den[NY][AW_NX];
+    // Numerators on the vector units, one per lane column; the
+    // denominators on the matrix cores.  For the tile row, den(px, d) is
+    // entry (px, c = px - d + 64) of the product of the reference rows
+    // (64 pixels x 2*PG taps, half) and the transposed target rows (128
+    // centres): summed over window rows, a 64 x 128 float matrix whose
+    // band c - px in [1, 64] holds every denominator of the tile.  Waves
+    // 0-5 each keep one 32 x 32 block of the band's six in accumulator
+    // registers (v_mfma_f32_32x32x8f16: products exact, float sums), K = 8
+    // taps per instruction; the zero taps that pad a row to PG words add
+    // nothing.
+    float num[AW_NX];
 #pragma unroll
-    for (int a = 0; a < NY; a++)
-#pragma unroll
-        for (int k = 0; k < AW_NX; k++) num[a][k] = den[a][k] = 0.0f;
+    for (int k = 0; k < AW_NX; k++) num[k] = 0.0f;
+    const bool has_tile = wave < 6;
+    const int mt = wave / 3, nt = mt + wave % 3;   // pixel block, centre block
+    f32x16 macc = {};
 
-    // The sums of window row yy from buffer BUF (the arithmetic of
-    // k_aw_band<true, R>).  Tap pair j of lane column k covers tile
-    // columns px+2j, px+2j+1 with px = 16*(wave>>1) + parity + 2k: floats
-    // 16*(wave>>1) + parity + 2(k+j) and the next of the lane's staged row.
+    // The sums of window row yy from buffer BUF.  Tap pair j of lane
+    // column k covers tile columns px+2j, px+2j+1 with px = 16*(wave>>1) +
+    // parity + 2k: floats 16*(wave>>1) + parity + 2(k+j) and the next of the
+    // lane's staged row.
     auto compute = [&](auto buf_tag, int yy) {
+        (void)yy;
         const uint32_t *buf = decltype(buf_tag)::value == 0 ? bufA : bufB;
         const float *cr = reinterpret_cast<const float *>(buf) + dl * PCF + 16 * (wave >> 1) + (wave & 1);
         h2 cw[AW_NX + R];
@@ -597,41 +610,44 @@ __global__ void __launch_bounds__(512) k_aw_band_dma(const uint32_t *__restrict_
             v.y = (_Float16)cr[2 * i + 1];
             cw[i] = v;
         }
-        const h2 one = {(_Float16)1.0f, (_Float16)1.0f};
-        const uint32_t *Lt = buf + T::L_OFF;
+        const uint32_t *Lt = buf + T::L_OFF, *Tt = buf + T::T_OFF;
+        if (has_tile) {
+            // lane l: pixel row / centre column l % 32, taps 4*(l/32) .. +3
+            // of each group of 8
+            const uint32_t *ar = Lt + (32 * mt + (dl & 31)) * PG + 2 * (dl >> 5);
+            const uint32_t *br = Tt + (32 * nt + (dl & 31)) * PG + 2 * (dl >> 5);
 #pragma unroll
-        for (int ny = 0; ny < NY; ny++) {
-            if (!row_valid(yy, ny)) continue;
-#pragma unroll
-            for (int k = 0; k < AW_NX; k++) {
-                const int px = aw_px<true>(wave, k);
-                const uint4 *tw = reinterpret_cast<const uint4 *>(buf + opaque(T::T_OFF + (ny * QW + px - dl + 64) * PG));
-                const uint4 *lw = reinterpret_cast<const uint4 *>(Lt + (ny * TXP + px) * PG);
-                uint4 tv[PG / 4], lv[PG / 4];
-#pragma unroll
-                for (int g = 0; g < PG / 4; g++) {
-                    tv[g] = tw[g];
-                    lv[g] = lw[g];
-                }
-                // The target row's last 16-byte word is read whole even where
-                // only its first words are taps: the lanes' rows lie PG words
-                // apart, and 4-byte reads at that stride share banks, where
-                // 16-byte reads do not.
-                if constexpr (NT % 4 != 0) asm volatile("" ::"v"(__builtin_bit_cast(u32x4, tv[PG / 4 - 1])));
-                const uint32_t *t32 = reinterpret_cast<const uint32_t *>(tv);
-                const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lv);
-                h2 p[NT];
-#pragma unroll
-                for (int j = 0; j < NT; j++) p[j] = __builtin_bit_cast(h2, l32[j]) * __builtin_bit_cast(h2, t32[j]);
-                float sn = num[ny][k], sd = den[ny][k];
-#pragma unroll
-                for (int j = 0; j < NT; j++) {
-                    sn = __builtin_amdgcn_fdot2(p[j], cw[k + j], sn, false);
-                    sd = __builtin_amdgcn_fdot2(p[j], one, sd, false);
-                }
-                num[ny][k] = sn;
-                den[ny][k] = sd;
+            for (int ks = 0; ks < PG / 4; ks++) {
+                const half4 av = *reinterpret_cast<const half4 *>(ar + 4 * ks);
+                const half4 bv = *reinterpret_cast<const half4 *>(br + 4 * ks);
+                macc = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, macc, 0, 0, 0);
             }
+        }
+#pragma unroll
+        for (int k = 0; k < AW_NX; k++) {
+            const int px = aw_px<true>(wave, k);
+            const uint4 *tw = reinterpret_cast<const uint4 *>(buf + opaque(T::T_OFF + (px - dl + 64) * PG));
+            const uint4 *lw = reinterpret_cast<const uint4 *>(Lt + px * PG);
+            uint4 tv[PG / 4], lv[PG / 4];
+#pragma unroll
+            for (int g = 0; g < PG / 4; g++) {
+                tv[g] = tw[g];
+                lv[g] = lw[g];
+            }
+            // The target row's last 16-byte word is read whole even where
+            // only its first words are taps: the lanes' rows lie PG words
+            // apart, and 4-byte reads at that stride share banks, where
+            // 16-byte reads do not.
+            if constexpr (NT % 4 != 0) asm volatile("" ::"v"(__builtin_bit_cast(u32x4, tv[PG / 4 - 1])));
+            const uint32_t *t32 = reinterpret_cast<const uint32_t *>(tv);
+            const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lv);
+            h2 p[NT];
+#pragma unroll
+            for (int j = 0; j < NT; j++) p[j] = __builtin_bit_cast(h2, l32[j]) * __builtin_bit_cast(h2, t32[j]);
+            float sn = num[k];
+#pragma unroll
+            for (int j = 0; j < NT; j++) sn = __builtin_amdgcn_fdot2(p[j], cw[k + j], sn, false);
+            num[k] = sn;
         }
     };
 
@@ -661,22 +677,32 @@ __global__ void __launch_bounds__(512) k_aw_band_dma(const uint32_t *__restrict_
         if (yy + 1 <= ylast) step(B{}, A{}, yy + 1);
     }
 
+    // The denominators: each wave's accumulator block into a 64 x 129
+    // float matrix (odd pitch: a lane's band entry px*129 + px - d + 64 is
+    // one word from its neighbour's), then each lane takes its pixels'.
+    float *dm = reinterpret_cast<float *>(bufB);
+    __syncthreads();
+    if (has_tile) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int i = 32 * mt + 8 * (q / 4) + 4 * (dl >> 5) + (q % 4), c = 32 * nt + (dl & 31);
+            dm[i * 129 + c] = macc[q];
+        }
+    }
+    __syncthreads();
     // Write through LDS so that each disparity row of 64 pixels is stored
     // contiguously.
     float *tile = reinterpret_cast<float *>(bufA);
 #pragma unroll
-    for (int ny = 0; ny < NY; ny++) {
-        if (ny >= nrows) break;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < AW_NX; k++) tile[dl * 65 + aw_px<true>(wave, k)] = num[ny][k] / den[ny][k];
-        __syncthreads();
-        const int y = yt + ny;
-        for (int i = tid; i < 64 * TXP; i += T::NTHR) {
-            const int dd = i / TXP, px = i % TXP;
-            const int d = d0 + dd, x = x0 + px;
-            if (d < D && x < W) stream_store(&out[(size_t)d * pl + (size_t)y * W + x], tile[dd * 65 + px]);
-        }
+    for (int k = 0; k < AW_NX; k++) {
+        const int px = aw_px<true>(wave, k);
+        tile[dl * 65 + px] = num[k] / dm[px * 129 + px - dl + 64];
+    }
+    __syncthreads();
+    for (int i = tid; i < 64 * TXP; i += T::NTHR) {
+        const int dd = i / TXP, px = i % TXP;
+        const int d = d0 + dd, x = x0 + px;
+        if (d < D && x < W) stream_store(&out[(size_t)d * pl + (size_t)yt * W + x], tile[dd * 65 + px]);
     }
 }
 
