@@ -69,12 +69,12 @@ TRAFFIC_SOURCE = "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/census_cross
 # Adaptive weights: 1089 taps per pd; 4 flops per tap (weight product, the
 # denominator's add, the numerator's multiply-add); the VALU issue count per
 # tap without packed FP32 is 3 (FP32: product, add, fused multiply-add) or
-# 1.5 (FP16: one packed product and two dot2 per tap pair); one VALU
-# instruction is 64 lanes, 256 CUs x 4 SIMDs issue one every 4 cycles at
-# 2.4 GHz.
+# 1 (FP16: one packed product and one dot2 per tap pair for the numerator;
+# the denominators run on the matrix cores); one VALU instruction is 64
+# lanes, 256 CUs x 4 SIMDs issue one every 4 cycles at 2.4 GHz.
 AW_TAPS = 33 * 33
 AW_FLOP_PER_TAP = 4.0
-AW_VALU_PER_TAP = {"fp32": 3.0, "fp16": 1.5}
+AW_VALU_PER_TAP = {"fp32": 3.0, "fp16": 1.0}
 PEAK_VECTOR_TFLOPS = 157.3
 VALU_LANE_OPS_PER_S = 256 * 4 * 16 * 2.4e9
 
