@@ -34,6 +34,7 @@ namespace rts {
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int AW_RMAX = 16;               // largest window radius
@@ -439,21 +440,25 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
 // staged conversion did.  The two buffers are separate LDS arrays: the
 // compiler then knows that the loads into one do not feed the reads of the
 // other and waits only as the code asks.
-template <int R, int NY>
+template <int R, int NY, bool C4>
 struct AwDma {
     static constexpr int NWAVE = 8, NTHR = 64 * NWAVE;
     static constexpr int TXP = 64, QW = TXP + 64;
     static constexpr int NT = R + 1, PG = (NT + 3) & ~3;
     // FP32 cost columns per disparity row: the floats a lane's pairs reach
-    // (up to 16*3 + 1 + 2*(AW_NX+R) - 1), at an odd pitch
-    static constexpr int PCF = (49 + 2 * (AW_NX + R)) | 1;
-    static constexpr int C_INS = (64 * PCF + 63) / 64;               // 4-byte loads of 64 lanes
+    // (up to 16*3 + 1 + 2*(AW_NX+R) - 1).  C4: rows of whole 16-byte words,
+    // loaded as such, at a pitch that is an odd multiple of 4 words (16-byte
+    // reads of the 64 lanes' rows then meet no bank conflicts); otherwise
+    // single words at an odd pitch.
+    static constexpr int NCF = 49 + 2 * (AW_NX + R);
+    static constexpr int PCF = C4 ? (((NCF + 3) / 4) | 1) * 4 : NCF | 1;
+    static constexpr int C_INS = C4 ? 64 * PCF / 256 : PCF;             // loads of 64 lanes
     static constexpr int T_QUADS = NY * QW * PG / 4, L_QUADS = NY * TXP * PG / 4;
     static constexpr int T_INS = (T_QUADS + 63) / 64, L_INS = (L_QUADS + 63) / 64;   // 16-byte loads
     static constexpr int C_PW = (C_INS + NWAVE - 1) / NWAVE, T_PW = (T_INS + NWAVE - 1) / NWAVE,
                          L_PW = (L_INS + NWAVE - 1) / NWAVE;
     static constexpr int NLOAD = C_PW + T_PW + L_PW;                   // loads per wave per window row
-    static constexpr int T_OFF = C_INS * 64, L_OFF = T_OFF + T_INS * 256;
+    static constexpr int T_OFF = 64 * PCF, L_OFF = T_OFF + T_INS * 256;
     // (each buffer also holds, at the end, the 64 x 129 denominator matrix
     // or the 64 x 65 output stage)
     static constexpr int WORDS = L_OFF + L_INS * 256 > 64 * 129 ? L_OFF + L_INS * 256 : 64 * 129;
@@ -487,12 +492,12 @@ __device__ __forceinline__ void wait_vmcnt() {
 }
 
 // (four waves per SIMD: two blocks per CU, so at most 128 registers)
-template <int R, int NY>
+template <int R, int NY, bool C4>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k_aw_band_dma(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb,
                                                      const float *__restrict__ vol, float *__restrict__ out, int W,
                                                      int H, int D, int Wp, int y0, int rows, int r, int nxb, int nyb,
                                                      int ndb) {
-    using T = AwDma<R, NY>;
+    using T = AwDma<R, NY, C4>;
     static_assert(NY == 1, "the matrix-core denominators cover one tile row");
     constexpr int NT = T::NT, PG = T::PG, TXP = T::TXP, QW = T::QW, PCF = T::PCF;
     __shared__ __attribute__((aligned(16))) uint32_t bufA[T::WORDS];
@@ -524,7 +529,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
     for (int k = 0; k < T::C_PW; k++) {
         const int ins = min(wave + T::NWAVE * k, T::C_INS - 1);
-        const int w = ins * 64 + dl;
+        const int w = (ins * 64 + dl) * (C4 ? 4 : 1);   // first staged word of the lane
         const int dd = min(w / PCF, 63), j = w - (w / PCF) * PCF;
         const int d = min(d0 + dd, D - 1);
         offc[k] = 4u * ((uint32_t)((size_t)(d - d0) * pl) + (uint32_t)wrap_any(x0 - R + j, W));
@@ -560,7 +565,10 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
 #pragma unroll
         for (int k = 0; k < T::C_PW; k++) {
             const int ins = min(wave + T::NWAVE * k, T::C_INS - 1);
-            dma_b32(crow, offc[k], lds_addr(buf + ins * 64));
+            if constexpr (C4)
+                dma_b128(crow, offc[k], lds_addr(buf + ins * 256));
+            else
+                dma_b32(crow, offc[k], lds_addr(buf + ins * 64));
         }
         const size_t i0 = wrow_index(yy, 0), i1 = wrow_index(yy, 1);
         const uint32_t step1 = (uint32_t)((i1 - i0) * (size_t)Wp * PG * 4);
@@ -584,9 +592,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     // centres): summed over window rows, a 64 x 128 float matrix whose
     // band c - px in [1, 64] holds every denominator of the tile.  Waves
     // 0-5 each keep one 32 x 32 block of the band's six in accumulator
-    // registers (v_mfma_f32_32x32x8f16: products exact, float sums), K = 8
-    // taps per instruction; the zero taps that pad a row to PG words add
-    // nothing.
+    // registers (v_mfma_f32_32x32x16_f16 and 32x32x8f16: products exact,
+    // float sums); the zero taps that pad a row to PG words add nothing.
     float num[AW_NX];
 #pragma unroll
     for (int k = 0; k < AW_NX; k++) num[k] = 0.0f;
@@ -601,25 +608,56 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     auto compute = [&](auto buf_tag, int yy) {
         (void)yy;
         const uint32_t *buf = decltype(buf_tag)::value == 0 ? bufA : bufB;
-        const float *cr = reinterpret_cast<const float *>(buf) + dl * PCF + 16 * (wave >> 1) + (wave & 1);
         h2 cw[AW_NX + R];
+        if constexpr (C4) {
+            // the lane's floats from 16*(wave>>1) as whole 16-byte words,
+            // then the pairs from the parity's offset
+            constexpr int NQ = (2 * (AW_NX + R) + 1 + 3) / 4;
+            const float4 *cq = reinterpret_cast<const float4 *>(buf + dl * PCF + 16 * (wave >> 1));
+            float cf[4 * NQ];
 #pragma unroll
-        for (int i = 0; i < AW_NX + R; i++) {
-            h2 v;
-            v.x = (_Float16)cr[2 * i];
-            v.y = (_Float16)cr[2 * i + 1];
-            cw[i] = v;
+            for (int q = 0; q < NQ; q++) {
+                const float4 v = cq[q];
+                cf[4 * q] = v.x;
+                cf[4 * q + 1] = v.y;
+                cf[4 * q + 2] = v.z;
+                cf[4 * q + 3] = v.w;
+            }
+            const bool par = wave & 1;
+#pragma unroll
+            for (int i = 0; i < AW_NX + R; i++) {
+                h2 v;
+                v.x = (_Float16)(par ? cf[2 * i + 1] : cf[2 * i]);
+                v.y = (_Float16)(par ? cf[2 * i + 2] : cf[2 * i + 1]);
+                cw[i] = v;
+            }
+        } else {
+            const float *cr = reinterpret_cast<const float *>(buf) + dl * PCF + 16 * (wave >> 1) + (wave & 1);
+#pragma unroll
+            for (int i = 0; i < AW_NX + R; i++) {
+                h2 v;
+                v.x = (_Float16)cr[2 * i];
+                v.y = (_Float16)cr[2 * i + 1];
+                cw[i] = v;
+            }
         }
         const uint32_t *Lt = buf + T::L_OFF, *Tt = buf + T::T_OFF;
         if (has_tile) {
-            // lane l: pixel row / centre column l % 32, taps 4*(l/32) .. +3
-            // of each group of 8
-            const uint32_t *ar = Lt + (32 * mt + (dl & 31)) * PG + 2 * (dl >> 5);
-            const uint32_t *br = Tt + (32 * nt + (dl & 31)) * PG + 2 * (dl >> 5);
+            // lane l: pixel row / centre column l % 32; taps in groups of 16
+            // (v_mfma_f32_32x32x16_f16, lane l: taps 8*(l/32) .. +7, whole
+            // 16-byte words at the rows' stride, no bank conflicts), then a
+            // last group of 8 where a row has 8 more (32x32x8, lane l: taps
+            // 4*(l/32) .. +3)
+            const uint32_t *ar = Lt + (32 * mt + (dl & 31)) * PG, *br = Tt + (32 * nt + (dl & 31)) * PG;
 #pragma unroll
-            for (int ks = 0; ks < PG / 4; ks++) {
-                const half4 av = *reinterpret_cast<const half4 *>(ar + 4 * ks);
-                const half4 bv = *reinterpret_cast<const half4 *>(br + 4 * ks);
+            for (int ks = 0; ks < PG / 8; ks++) {
+                const half8 av = *reinterpret_cast<const half8 *>(ar + 8 * ks + 4 * (dl >> 5));
+                const half8 bv = *reinterpret_cast<const half8 *>(br + 8 * ks + 4 * (dl >> 5));
+                macc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, macc, 0, 0, 0);
+            }
+            if constexpr (PG % 8 != 0) {
+                const half4 av = *reinterpret_cast<const half4 *>(ar + PG - 4 + 2 * (dl >> 5));
+                const half4 bv = *reinterpret_cast<const half4 *>(br + PG - 4 + 2 * (dl >> 5));
                 macc = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, macc, 0, 0, 0);
             }
         }
@@ -713,12 +751,18 @@ int aw_band_launch(const void *wl, const void *wt, const float *vol, float *out,
     // planes in 32 bits
     if (HALF && 64.0 * (double)W * (double)H < 4294967296.0) {
         // one tile row per block: two blocks (two double buffers) share a CU
+        // (the cost rows as 16-byte words where their columns x0-R+4i are
+        // 16-byte aligned in every plane and across the wrap)
         constexpr int NY = 1;
-        using TD = AwDma<R, NY>;
+        using TD = AwDma<R, NY, false>;
         const int nxb = div_up(W, TD::TXP), nyb = div_up(rows, NY), ndb = div_up(D, 64);
-        k_aw_band_dma<R, NY><<<nxb * nyb * ndb, TD::NTHR, 0, s>>>(static_cast<const uint32_t *>(wl),
-                                                                   static_cast<const uint32_t *>(wt), vol, out, W, H,
-                                                                   D, Wp, y0, rows, r, nxb, nyb, ndb);
+        const uint32_t *wlp = static_cast<const uint32_t *>(wl), *wtp = static_cast<const uint32_t *>(wt);
+        if (R % 4 == 0 && W % 4 == 0 && (reinterpret_cast<uintptr_t>(vol) & 15) == 0)
+            k_aw_band_dma<R, NY, true><<<nxb * nyb * ndb, TD::NTHR, 0, s>>>(wlp, wtp, vol, out, W, H, D, Wp, y0,
+                                                                          rows, r, nxb, nyb, ndb);
+        else
+            k_aw_band_dma<R, NY, false><<<nxb * nyb * ndb, TD::NTHR, 0, s>>>(wlp, wtp, vol, out, W, H, D, Wp, y0,
+                                                                           rows, r, nxb, nyb, ndb);
         RTS_LAUNCHED();
         return RTS_OK;
     }
