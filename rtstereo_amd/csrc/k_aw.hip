@@ -432,14 +432,13 @@ __global__ void __launch_bounds__(512) k_aw_band(const uint32_t *__restrict__ wl
 
 // FP16 aggregation with its staging by direct loads into LDS (global_load_lds:
 // no registers, no LDS store instructions), double-buffered so that window
-// row yy+1 arrives while row yy is summed.  Same tile (64 pixels x 2 rows x
-// 64 disparities, lane = disparity, 8 waves) and the same arithmetic as
-// k_aw_band<true, R>; the cost rows stay FP32 in LDS (an odd row pitch, so
-// that the per-lane row reads meet no bank conflicts) and each lane packs its
-// register window of half pairs with v_cvt_pk_f16_f32, which rounds as the
-// staged conversion did.  The two buffers are separate LDS arrays: the
-// compiler then knows that the loads into one do not feed the reads of the
-// other and waits only as the code asks.
+// row yy+1 arrives while row yy is summed.  A block is one tile row of 64
+// pixels x 64 disparities (lane = disparity, 8 waves; two blocks share a CU);
+// the numerators use the arithmetic of k_aw_band<true, R> (half products,
+// dot2 into float), the denominators run on the matrix cores.  The cost rows
+// stay FP32 in LDS (a pitch at which the per-lane row reads meet no bank
+// conflicts) and each lane packs its register window of half pairs with
+// v_cvt_pk_f16_f32, which rounds as the staged conversion did.
 template <int R, int NY, bool C4>
 struct AwDma {
     static constexpr int NWAVE = 8, NTHR = 64 * NWAVE;
@@ -500,8 +499,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     using T = AwDma<R, NY, C4>;
     static_assert(NY == 1, "the matrix-core denominators cover one tile row");
     constexpr int NT = T::NT, PG = T::PG, TXP = T::TXP, QW = T::QW, PCF = T::PCF;
-    __shared__ __attribute__((aligned(16))) uint32_t bufA[T::WORDS];
-    __shared__ __attribute__((aligned(16))) uint32_t bufB[T::WORDS];
+    __shared__ __attribute__((aligned(16))) uint32_t lbuf[2 * T::WORDS];
+    uint32_t *const bufA = lbuf, *const bufB = lbuf + T::WORDS;
 
     const int nblk = nxb * nyb * ndb;
     const int lin = xcd_order(blockIdx.x, nblk);
@@ -559,8 +558,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     // The weight rows of tile row 1 start (row 1's index - row 0's) rows of
     // Wp*PG words after row 0's; that distance is added to the lanes whose
     // load belongs to tile row 1 (uniform per window row).
-    auto issue = [&](auto buf_tag, int yy) {
-        const uint32_t *buf = decltype(buf_tag)::value == 0 ? bufA : bufB;
+    auto issue = [&](int which, int yy) {
+        const uint32_t *buf = lbuf + which * T::WORDS;
         const float *crow = vol + (size_t)d0 * pl + (size_t)wrap_any(yy, H) * W;
 #pragma unroll
         for (int k = 0; k < T::C_PW; k++) {
@@ -605,9 +604,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     // column k covers tile columns px+2j, px+2j+1 with px = 16*(wave>>1) +
     // parity + 2k: floats 16*(wave>>1) + parity + 2(k+j) and the next of the
     // lane's staged row.
-    auto compute = [&](auto buf_tag, int yy) {
-        (void)yy;
-        const uint32_t *buf = decltype(buf_tag)::value == 0 ? bufA : bufB;
+    auto compute = [&](int which) {
+        const uint32_t *buf = lbuf + which * T::WORDS;
         h2 cw[AW_NX + R];
         if constexpr (C4) {
             // the lane's floats from 16*(wave>>1) as whole 16-byte words,
@@ -689,31 +687,26 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
         }
     };
 
-    using A = std::integral_constant<int, 0>;
-    using B = std::integral_constant<int, 1>;
     const int ylast = yt + nrows - 1 + r;
     // Each step: every wave is done reading the buffer about to be refilled
     // (its reads have returned: lgkmcnt 0, then the barrier); the next row's
     // loads go out; this wave's loads of the current row are waited for
     // (vmcnt = the loads just issued) and the barrier makes every wave's part
     // of the buffer complete.
-    auto step = [&](auto cur, auto nxt, int yy) {
+    auto step = [&](int cur, int yy) {
         __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
         __builtin_amdgcn_s_barrier();
         if (yy < ylast) {
-            issue(nxt, yy + 1);
+            issue(cur ^ 1, yy + 1);
             wait_vmcnt<T::NLOAD>();
         } else {
             wait_vmcnt<0>();
         }
         __builtin_amdgcn_s_barrier();
-        compute(cur, yy);
+        compute(cur);
     };
-    issue(A{}, yt - r);
-    for (int yy = yt - r; yy <= ylast; yy += 2) {
-        step(A{}, B{}, yy);
-        if (yy + 1 <= ylast) step(B{}, A{}, yy + 1);
-    }
+    issue(0, yt - r);
+    for (int yy = yt - r; yy <= ylast; yy++) step((yy - yt + r) & 1, yy);
 
     // The denominators: each wave's accumulator block into a 64 x 129
     // float matrix (odd pitch: a lane's band entry px*129 + px - d + 64 is
