@@ -247,6 +247,24 @@ def test_aw_bands_full_width():
             assert np.max(np.abs(got - f64)) < 0.05
 
 
+@pytest.mark.parametrize("W,H,D,r", [(128, 40, 100, 8), (132, 36, 70, 5), (130, 40, 64, 16), (256, 40, 192, 16)])
+def test_aw_fp16_kernel_paths(W, H, D, r):
+    """The FP16 aggregation's forms: cost rows staged as 16-byte words (r 8
+    and 16 at widths divisible by 4) or as single words (other radii or
+    widths), disparity blocks cut short (D 70, 100), several column tiles and
+    disparity blocks with the wrap at both borders (256 x 40, D 192); every
+    pixel and disparity against the FP64 oracle."""
+    print(f"[case] aw fp16 paths {W}x{H} D={D} r={r}", flush=True)
+    p = gen.make_pair(W, H, D, 8, 3.0, 31)
+    ll, lr = p.lab()
+    c = oracle.ad_color(p.left, p.right, D, 1, 40.0, False)
+    ys, xs = np.mgrid[0:H, 0:W]
+    f64 = oracle.aw_l2r_f64(ll, lr, c, ys.ravel(), xs.ravel(), r, 7.0, 36.0).reshape(H, W, D)
+    got16 = pm(S.aw(ll, lr, dm(c), r, 7.0, 36.0, L.RTS_AW_FP16)).astype(np.float64)
+    assert np.isfinite(got16).all()
+    assert np.max(np.abs(got16 - f64)) < 0.05
+
+
 @pytest.mark.parametrize("Lm,tao", [(40, 30), (60, 255)])
 def test_cross_long_arms(Lm, tao):
     """Arms longer than 17 (beyond the sweep's ring window) take the sweep's
