@@ -395,19 +395,31 @@ def main() -> int:
     achieved = bytes_launch / (stage[dom] / 1e3) / 1e9
 
     copy = None
+    probes = None
     if not args.no_copy_probe:
+        # 2 GiB copy, read and write probes (16-byte accesses, 8 in flight
+        # per lane): the achievable rates the frame's bytes are held to
         n = 1 << 31
         a, b = DeviceArray((n,), np.uint8), DeviceArray((n,), np.uint8)
-        e0, e1 = Event(), Event()
-        call("rts_copy_kernel", a.ptr, b.ptr, n, stream.handle)
-        e0.record(stream.handle)
-        for _ in range(10):
-            call("rts_copy_kernel", a.ptr, b.ptr, n, stream.handle)
-        e1.record(stream.handle)
-        stream.sync()
-        copy = 2 * n * 10 / (e0.elapsed_ms(e1) / 1e3) / 1e9
+        sink = DeviceArray((4096,), np.uint32)
+
+        def rate(fn, bytes_moved):
+            fn()
+            e0, e1 = Event(), Event()
+            e0.record(stream.handle)
+            for _ in range(10):
+                fn()
+            e1.record(stream.handle)
+            stream.sync()
+            return bytes_moved * 10 / (e0.elapsed_ms(e1) / 1e3) / 1e9
+
+        copy = rate(lambda: call("rts_copy_kernel", a.ptr, b.ptr, n, stream.handle), 2 * n)
+        probes = {"copy": copy,
+                  "read": rate(lambda: call("rts_read_kernel", a.ptr, n, sink.ptr, stream.handle), n),
+                  "write": rate(lambda: call("rts_write_kernel", b.ptr, n, stream.handle), n)}
         a.free()
         b.free()
+        sink.free()
 
     out = {
         "metric": METRIC,
@@ -452,7 +464,15 @@ def main() -> int:
                                       "frac": FRAME_CONTRACT_BYTES_PER_PD * pd / (total_ms / args.steps / 1e3) / 1e9
                                       / PEAK_HBM_GBS},
         "copy_kernel_gbs": copy,
+        "bandwidth_probes_gbs": probes,
     }
+    if probes:
+        # The frame's floor on this GPU's measured rates: it reads 8 B/pd
+        # (prefix and aggregated volumes) and writes 8 B/pd (the same two).
+        floor_ms = (8.0 * pd / (probes["read"] * 1e9) + 8.0 * pd / (probes["write"] * 1e9)) * 1e3
+        out["roofline_frame_measured"] = {"read_bytes_per_pd": 8.0, "write_bytes_per_pd": 8.0,
+                                          "read_gbs": probes["read"], "write_gbs": probes["write"],
+                                          "floor_ms": floor_ms, "frac": floor_ms / (total_ms / args.steps)}
     if not args.no_aw:
         progress("adaptive weights FP32 / FP16")
         out["aw"] = aw_timing(stream, AW_ROWS if args.cpu_seconds > 0 else None)

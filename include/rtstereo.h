@@ -87,6 +87,13 @@ rts_status rts_vol_to_pm(const float *dm, float *pm, int W, int H, int D, void *
  * counters and the achievable copy bandwidth. */
 rts_status rts_copy_kernel(const void *src, void *dst, size_t bytes, void *stream);
 
+/* Read-only and write-only kernels of the same shape over `bytes` (multiple
+ * of 16): the achievable read and write bandwidths, which bound a stage by
+ * the bytes it reads and writes.  The read probe may store into `sink`
+ * (4096 words).  No reference counterpart (measurement). */
+rts_status rts_read_kernel(const void *src, size_t bytes, void *sink, void *stream);
+rts_status rts_write_kernel(void *dst, size_t bytes, void *stream);
+
 /* ------------------------------------------------------------ matching costs */
 
 /* Direction of the target pixel: L2R pairs ref x with tar x-d, R2L with x+d;

@@ -63,6 +63,8 @@ _SIGS = {
     "rts_mem_info": [C.POINTER(sz), C.POINTER(sz)],
     "rts_vol_to_dm": [vp, vp, i32, i32, i32, vp], "rts_vol_to_pm": [vp, vp, i32, i32, i32, vp],
     "rts_copy_kernel": [vp, vp, sz, vp],
+    "rts_read_kernel": [vp, sz, vp, vp],
+    "rts_write_kernel": [vp, sz, vp],
     "rts_ad_gray": [vp, vp, vp, i32, i32, i32, i32, vp],
     "rts_ad_float": [vp, vp, vp, i32, i32, i32, i32, vp],
     "rts_ad_color": [vp, vp, vp, i32, i32, i32, i32, i32, f32, i32, vp],
