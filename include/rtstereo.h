@@ -94,6 +94,12 @@ rts_status rts_copy_kernel(const void *src, void *dst, size_t bytes, void *strea
 rts_status rts_read_kernel(const void *src, size_t bytes, void *sink, void *stream);
 rts_status rts_write_kernel(void *dst, size_t bytes, void *stream);
 
+/* A float volume of n values (a multiple of 8; both 16-byte aligned) as
+ * half, each value rounded to nearest: the FP16 cost volume that
+ * rts_aw_half_ws reads.  No reference counterpart (the reference's costs
+ * are float). */
+rts_status rts_volume_to_half(const float *src, void *dst, size_t n, void *stream);
+
 /* ------------------------------------------------------------ matching costs */
 
 /* Direction of the target pixel: L2R pairs ref x with tar x-d, R2L with x+d;
@@ -111,6 +117,10 @@ rts_status rts_ad_float(const float *ref, const float *tar, float *vol, int W, i
  * pixelBasedCostL2R_Color (both overloads) / pixelBasedR2L_Color. */
 rts_status rts_ad_color(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int dir,
                         int use_trunc, float trunc, int norm, void *stream);
+/* The same costs written as half (each float cost rounded to nearest), the
+ * input of rts_aw_half_ws. */
+rts_status rts_ad_color_half(const uint8_t *ref, const uint8_t *tar, void *vol_half, int W, int H, int D, int dir,
+                             int use_trunc, float trunc, int norm, void *stream);
 
 /* Census descriptors packed into uint64 words ([nw][H][W], nw = ceil(N/64),
  * N = (2hr+1)(2wr+1)).  seq/CensusTransform.cpp:11-37 calcCensusTrans. */
@@ -227,6 +237,19 @@ size_t rts_aw_workspace_bytes(int W, int H, int r, int precision);
 rts_status rts_aw_ws(const uint8_t *lab_ref, const uint8_t *lab_tar, const float *vol, float *out, int W, int H,
                      int D, int r, float gamma_c, float gamma_p, int precision, void *workspace,
                      size_t workspace_bytes, void *stream);
+
+/* RTS_AW_FP16 aggregation (seq/AdaptiveWeightAggregation.cpp:73-111
+ * calcAWCostL2R with the weights of :11-71) from an FP16 cost volume
+ * (half [D][H][W], each the float cost rounded to nearest, 16-byte
+ * aligned): bit for bit the RTS_AW_FP16 result of rts_aw_ws on the float
+ * volume, with up to three disparities per lane so that each weight row
+ * read from LDS serves several outputs.  Needs W a multiple of 8 and
+ * 384*W*H < 2^32 (rts_aw_half_supported); the workspace is
+ * rts_aw_workspace_bytes(W, H, r, RTS_AW_FP16). */
+int rts_aw_half_supported(int W, int H, int D);
+rts_status rts_aw_half_ws(const uint8_t *lab_ref, const uint8_t *lab_tar, const void *vol_half, float *out, int W,
+                          int H, int D, int r, float gamma_c, float gamma_p, void *workspace, size_t workspace_bytes,
+                          void *stream);
 
 /* Cross arms (left, right, top, bottom), at most L, colour threshold tao.
  * seq/CrossBasedAggregation.cpp:12-115 calc_x_range / calc_y_range / findCross. */

@@ -65,9 +65,11 @@ _SIGS = {
     "rts_copy_kernel": [vp, vp, sz, vp],
     "rts_read_kernel": [vp, sz, vp, vp],
     "rts_write_kernel": [vp, sz, vp],
+    "rts_volume_to_half": [vp, vp, sz, vp],
     "rts_ad_gray": [vp, vp, vp, i32, i32, i32, i32, vp],
     "rts_ad_float": [vp, vp, vp, i32, i32, i32, i32, vp],
     "rts_ad_color": [vp, vp, vp, i32, i32, i32, i32, i32, f32, i32, vp],
+    "rts_ad_color_half": [vp, vp, vp, i32, i32, i32, i32, i32, f32, i32, vp],
     "rts_census": [vp, vp, i32, i32, i32, i32, vp],
     "rts_census_unpack": [vp, vp, i32, i32, i32, vp], "rts_census_pack": [vp, vp, i32, i32, i32, vp],
     "rts_hamming": [vp, vp, vp, i32, i32, i32, i32, vp],
@@ -82,6 +84,7 @@ _SIGS = {
     "rts_aw_weights": [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp],
     "rts_aw": [vp, vp, vp, vp, i32, i32, i32, i32, f32, f32, i32, vp],
     "rts_aw_ws": [vp, vp, vp, vp, i32, i32, i32, i32, f32, f32, i32, vp, sz, vp],
+    "rts_aw_half_ws": [vp, vp, vp, vp, i32, i32, i32, i32, f32, f32, vp, sz, vp],
     "rts_find_cross": [vp, vp, i32, i32, i32, i32, vp],
     "rts_arms_unpack": [vp, vp, vp, i32, i32, vp], "rts_arms_pack": [vp, vp, vp, i32, i32, vp],
     "rts_hprefix": [vp, i32, i32, i32, vp],
@@ -95,6 +98,7 @@ _SIGS = {
     "rts_pipeline_defaults": [C.POINTER(PipelineDesc), i32, i32, i32],
     "rts_pipeline_bytes": [C.POINTER(PipelineDesc)],
     "rts_aw_workspace_bytes": [i32, i32, i32, i32],
+    "rts_aw_half_supported": [i32, i32, i32],
     "rts_cost_rank_log_ncc": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, vp],
     "rts_cost_rank_log_ncc_dch": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, i32, vp],
     "rts_cost_rank_log_ncc_ws": [vp, vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, f32, f32, f32, i32, vp, sz, vp],

@@ -456,13 +456,11 @@ struct AwDma {
     static constexpr int T_INS = (T_QUADS + 63) / 64, L_INS = (L_QUADS + 63) / 64;   // 16-byte loads
     static constexpr int C_PW = (C_INS + NWAVE - 1) / NWAVE, T_PW = (T_INS + NWAVE - 1) / NWAVE,
                          L_PW = (L_INS + NWAVE - 1) / NWAVE;
-    static constexpr int NLOAD = C_PW + T_PW + L_PW;                   // loads per wave per window row
     static constexpr int T_OFF = 64 * PCF, L_OFF = T_OFF + T_INS * 256;
     // (each buffer also holds, at the end, the 64 x 129 denominator matrix
     // or the 64 x 65 output stage)
     static constexpr int WORDS = L_OFF + L_INS * 256 > 64 * 129 ? L_OFF + L_INS * 256 : 64 * 129;
     static_assert(2 * WORDS * 4 <= 160 * 1024, "two buffers must fit the LDS");
-    static_assert(NLOAD < 64, "vmcnt counts at most 63");
 };
 
 // Direct loads into LDS (64 lanes: lane i's 4 or 16 bytes from base + its
@@ -688,21 +686,15 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
     };
 
     const int ylast = yt + nrows - 1 + r;
-    // Each step: every wave is done reading the buffer about to be refilled
-    // (its reads have returned: lgkmcnt 0, then the barrier); the next row's
-    // loads go out; this wave's loads of the current row are waited for
-    // (vmcnt = the loads just issued) and the barrier makes every wave's part
-    // of the buffer complete.
+    // Each step: every wave's reads of the previous row have returned
+    // (lgkmcnt 0) and its loads of this row have landed (vmcnt 0); past the
+    // barrier the row is complete in the buffer and the other buffer is free
+    // for the next row's loads, which fly during this row's sums.
     auto step = [&](int cur, int yy) {
         __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+        wait_vmcnt<0>();
         __builtin_amdgcn_s_barrier();
-        if (yy < ylast) {
-            issue(cur ^ 1, yy + 1);
-            wait_vmcnt<T::NLOAD>();
-        } else {
-            wait_vmcnt<0>();
-        }
-        __builtin_amdgcn_s_barrier();
+        if (yy < ylast) issue(cur ^ 1, yy + 1);
         compute(cur);
     };
     issue(0, yt - r);
@@ -735,6 +727,309 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) k
         const int d = d0 + dd, x = x0 + px;
         if (d < D && x < W) stream_store(&out[(size_t)d * pl + (size_t)yt * W + x], tile[dd * 65 + px]);
     }
+}
+
+// FP16 aggregation from an FP16 cost volume, M disparities per lane.  A block
+// is one tile row of 64 pixels x 64*M disparities (8 waves).  Lane l owns
+// disparities b, b+2, .., b+2(M-1) with b = 2M*(l/2) + l%2, and a wave walks
+// the columns px, px+2, .., so the target centre of (px, b+2j) is the one of
+// (px-2j, b): each column brings one new target weight row per lane, the
+// other M-1 come from the previous columns' registers, and the column's
+// reference row serves all M outputs.  Against one disparity per lane this
+// reads 1/M of the weight bytes from LDS per output.  The cost rows come
+// from the FP16 volume (whole 16-byte words, staged from column x0 - R8);
+// each lane forms its tap pairs with one byte permute per pair.  Products,
+// sums, their order and the matrix-core denominators are those of
+// k_aw_band_dma, so the two kernels' outputs are bit-identical when the
+// FP16 volume holds the FP32 costs rounded to nearest.
+template <int R, int M>
+struct AwHalf {
+    static constexpr int NWAVE = 8, NTHR = 64 * NWAVE;
+    static constexpr int TXP = 64, DT = 64 * M, QW = TXP + DT;
+    static constexpr int NT = R + 1, PG = (NT + 3) & ~3;
+    static constexpr int R8 = (R + 7) & ~7;   // the staged row starts at x0 - R8 (16-byte aligned)
+    // halves of a cost row that the lanes reach: from x0 - R8 up to column
+    // 16*3 + 1 + 2*(AW_NX-1) + R + 1 of the tile
+    static constexpr int NCH = R8 - R + 49 + 2 * (AW_NX + R);
+    static constexpr int PCQ = ((NCH + 7) / 8) | 1;   // row pitch in 16-byte words, odd
+    static constexpr int PCW = 4 * PCQ;
+    // 16-byte words a lane reads of one row (from word 2*(wave/2))
+    static constexpr int NQ = (R8 - R + 1 + 2 * (AW_NX + R) + 7) / 8;
+    static constexpr int C_INS = DT * PCQ / 64;          // 16-byte loads of 64 lanes
+    static constexpr int T_INS = QW * PG / 4 / 64, L_INS = TXP * PG / 4 / 64;
+    static constexpr int C_PW = (C_INS + NWAVE - 1) / NWAVE, T_PW = (T_INS + NWAVE - 1) / NWAVE,
+                         L_PW = (L_INS + NWAVE - 1) / NWAVE;
+    static constexpr int T_OFF = DT * PCW, L_OFF = T_OFF + QW * PG, WORDS = L_OFF + TXP * PG;
+    static constexpr int DMP = QW + 1;                   // pitch of the denominator matrix (odd)
+    static constexpr int NBLK = 2 * (2 * M + 1), NSLOT = (NBLK + NWAVE - 1) / NWAVE;
+    static constexpr int LBUF_A = 2 * WORDS > 64 * DMP ? 2 * WORDS : 64 * DMP;
+    static constexpr int LBUF = LBUF_A > DT * 65 ? LBUF_A : DT * 65;
+    static_assert(QW * PG % 256 == 0 && TXP * PG % 256 == 0 && DT * PCQ % 64 == 0, "whole loads");
+    static_assert(6 + NQ <= PCQ, "a lane's words lie in its row");
+    static_assert(((R8 - R) >> 1) + AW_NX + R < 4 * NQ, "a lane's pairs lie in its words");
+    static_assert(LBUF * 4 <= 160 * 1024, "the buffers must fit the LDS");
+};
+
+template <int R, int M>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(M == 1 ? 4 : 2)))
+k_aw_band_h(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb, const uint16_t *__restrict__ volh,
+            float *__restrict__ out, int W, int H, int D, int Wp, int y0, int r, int nxb, int nyb, int ndb) {
+    using T = AwHalf<R, M>;
+    constexpr int NT = T::NT, PG = T::PG, DT = T::DT, PCW = T::PCW, PCQ = T::PCQ, DMP = T::DMP;
+    __shared__ __attribute__((aligned(16))) uint32_t lbuf[T::LBUF];
+
+    const int nblk = nxb * nyb * ndb;
+    const int lin = xcd_order(blockIdx.x, nblk);
+    const int xb = lin % nxb, rest = lin / nxb;
+    const int db = rest % ndb, yb = rest / ndb;
+    const int x0 = xb * 64, d0 = db * DT, yt = y0 + yb;
+    const int tid = threadIdx.x, dl = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int e = 2 * r + 1;
+    const size_t pl = plane(W, H);
+    const int q_lo = x0 - d0 - DT;   // target centre of LDS row 0
+
+    // Per-lane byte offsets of the loads, fixed for the block.
+    uint32_t offc[T::C_PW], offt[T::T_PW], offl[T::L_PW];
+#pragma unroll
+    for (int k = 0; k < T::C_PW; k++) {
+        const int u = min(wave + T::NWAVE * k, T::C_INS - 1) * 64 + dl;   // 16-byte word of the tile's rows
+        const int dd = u / PCQ, j = u - dd * PCQ;
+        const int d = min(d0 + dd, D - 1);
+        offc[k] = 2u * ((uint32_t)((size_t)(d - d0) * pl) + (uint32_t)wrap_any(x0 - T::R8 + 8 * j, W));
+    }
+#pragma unroll
+    for (int k = 0; k < T::T_PW; k++) {
+        const int u = min(wave + T::NWAVE * k, T::T_INS - 1) * 64 + dl;
+        const int q = u / (PG / 4), g = u - q * (PG / 4);
+        offt[k] = 4u * ((uint32_t)wrap_any(q_lo + q, W) * PG + 4 * g);
+    }
+#pragma unroll
+    for (int k = 0; k < T::L_PW; k++) {
+        const int u = min(wave + T::NWAVE * k, T::L_INS - 1) * 64 + dl;
+        const int px = u / (PG / 4), g = u - px * (PG / 4);
+        offl[k] = 4u * ((uint32_t)(x0 + px) * PG + 4 * g);
+    }
+
+    // Window row yy into buffer WHICH: the FP16 cost rows of the block's DT
+    // disparities (columns x0-R8 .., wrapped), the target weight rows of the
+    // QW centres from q_lo and the reference rows of the tile's pixels.
+    auto issue = [&](int which, int yy) {
+        const uint32_t *buf = lbuf + which * T::WORDS;
+        const uint16_t *crow = volh + (size_t)d0 * pl + (size_t)wrap_any(yy, H) * W;
+#pragma unroll
+        for (int k = 0; k < T::C_PW; k++)
+            dma_b128(crow, offc[k], lds_addr(buf + min(wave + T::NWAVE * k, T::C_INS - 1) * 256));
+        const size_t wr = (size_t)yb * e + (yy - yt + r);
+        const uint32_t *tb = wtb + wr * Wp * PG, *lb = wlb + wr * Wp * PG;
+#pragma unroll
+        for (int k = 0; k < T::T_PW; k++)
+            dma_b128(tb, offt[k], lds_addr(buf + T::T_OFF + min(wave + T::NWAVE * k, T::T_INS - 1) * 256));
+#pragma unroll
+        for (int k = 0; k < T::L_PW; k++)
+            dma_b128(lb, offl[k], lds_addr(buf + T::L_OFF + min(wave + T::NWAVE * k, T::L_INS - 1) * 256));
+    };
+
+    const int base = 2 * M * (dl >> 1) + (dl & 1);   // the lane's first disparity in the tile
+    const int pxw = 16 * (wave >> 1) + (wave & 1);    // the wave's first column
+    // Tap pair i of column k and disparity row j: halves
+    // R8 - R + (wave & 1) + 2(k + i) and the next of the lane's words from
+    // 16-byte word 2*(wave/2) of the row; f halves past word R8-R>>1 + k + i.
+    constexpr int CB = (T::R8 - R) >> 1;
+    const int f = ((T::R8 - R) & 1) + (wave & 1);
+    const uint32_t psel = f == 0 ? 0x03020100u : f == 1 ? 0x05040302u : 0x07060504u;
+    float num[AW_NX][M];
+#pragma unroll
+    for (int k = 0; k < AW_NX; k++)
+#pragma unroll
+        for (int j = 0; j < M; j++) num[k][j] = 0.0f;
+    f32x16 macc[T::NSLOT];
+#pragma unroll
+    for (int s = 0; s < T::NSLOT; s++) macc[s] = f32x16{};
+
+    auto compute = [&](int which, int yy) {
+        const uint32_t *buf = lbuf + which * T::WORDS;
+        uint32_t cw[M][AW_NX + R];
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            const u32x4 *cq = reinterpret_cast<const u32x4 *>(buf + (base + 2 * j) * PCW + 8 * (wave >> 1));
+            uint32_t wd[4 * T::NQ];
+#pragma unroll
+            for (int q = 0; q < T::NQ; q++) {
+                const u32x4 v = cq[q];
+                wd[4 * q] = v.x;
+                wd[4 * q + 1] = v.y;
+                wd[4 * q + 2] = v.z;
+                wd[4 * q + 3] = v.w;
+            }
+#pragma unroll
+            for (int i = 0; i < AW_NX + R; i++)
+                cw[j][i] = __builtin_amdgcn_perm(wd[CB + i + 1], wd[CB + i], psel);
+        }
+        const uint32_t *Lt = buf + T::L_OFF, *Tt = buf + T::T_OFF;
+        // denominators: block b = wave + 8s of the band's NBLK, pixel block
+        // mt, centre block nt (as in k_aw_band_dma)
+#pragma unroll
+        for (int s = 0; s < T::NSLOT; s++) {
+            const int b = wave + T::NWAVE * s;
+            if (b < T::NBLK) {
+                const int mt = b / (2 * M + 1), nt = mt + b % (2 * M + 1);
+                const uint32_t *ar = Lt + (32 * mt + (dl & 31)) * PG, *br = Tt + (32 * nt + (dl & 31)) * PG;
+#pragma unroll
+                for (int ks = 0; ks < PG / 8; ks++) {
+                    const half8 av = *reinterpret_cast<const half8 *>(ar + 8 * ks + 4 * (dl >> 5));
+                    const half8 bv = *reinterpret_cast<const half8 *>(br + 8 * ks + 4 * (dl >> 5));
+                    macc[s] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, macc[s], 0, 0, 0);
+                }
+                if constexpr (PG % 8 != 0) {
+                    const half4 av = *reinterpret_cast<const half4 *>(ar + PG - 4 + 2 * (dl >> 5));
+                    const half4 bv = *reinterpret_cast<const half4 *>(br + PG - 4 + 2 * (dl >> 5));
+                    macc[s] = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, macc[s], 0, 0, 0);
+                }
+            }
+        }
+        // numerators: target row c_k = pxw + 2k - base + DT for k from
+        // -(M-1); output (k, j) uses c_(k-j)
+        uint32_t tv[AW_NX + M][NT];
+        // one per-lane base; the rows of the columns at immediate offsets
+        const uint32_t *tb = buf + T::T_OFF + opaque((pxw - 2 * (M - 1) - base + DT) * PG);
+        auto load_t = [&](int kk) {
+            const uint32_t *tr = tb + 2 * kk * PG;
+#pragma unroll
+            for (int g = 0; g < NT / 4; g++) {
+                const uint4 v = reinterpret_cast<const uint4 *>(tr)[g];
+                tv[kk][4 * g] = v.x;
+                tv[kk][4 * g + 1] = v.y;
+                tv[kk][4 * g + 2] = v.z;
+                tv[kk][4 * g + 3] = v.w;
+            }
+#pragma unroll
+            for (int i = 4 * (NT / 4); i < NT; i++) tv[kk][i] = tr[i];
+        };
+        // reference row of column k (its NT words)
+        uint32_t lv[AW_NX][NT];
+        auto load_l = [&](int k) {
+            const uint4 *lw = reinterpret_cast<const uint4 *>(Lt + (pxw + 2 * k) * PG);
+#pragma unroll
+            for (int g = 0; g < NT / 4; g++) {
+                const uint4 v = lw[g];
+                lv[k][4 * g] = v.x;
+                lv[k][4 * g + 1] = v.y;
+                lv[k][4 * g + 2] = v.z;
+                lv[k][4 * g + 3] = v.w;
+            }
+#pragma unroll
+            for (int i = 4 * (NT / 4); i < NT; i++) lv[k][i] = Lt[(pxw + 2 * k) * PG + i];
+        };
+        // software pipeline: column k+1's target row (M > 1) and reference
+        // row (M = 2) are read while column k is summed, as registers permit
+        // (the scheduling barrier keeps the reads ahead of the sums)
+        constexpr bool PRE_T = M > 1, PRE_L = M == 2;
+#pragma unroll
+        for (int kk = 0; kk < (PRE_T ? M : M - 1); kk++) load_t(kk);
+        if (PRE_L) load_l(0);
+#pragma unroll
+        for (int k = 0; k < AW_NX; k++) {
+            if (!PRE_T) load_t(k + M - 1);
+            if (!PRE_L) load_l(k);
+            if (PRE_T && k + 1 < AW_NX) {
+                load_t(k + M);
+                if (PRE_L) load_l(k + 1);
+            }
+            if (PRE_T) __builtin_amdgcn_sched_barrier(0);
+            const uint32_t *l32 = lv[k];
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                const uint32_t *t32 = tv[k + M - 1 - j];
+                h2 p[NT];
+#pragma unroll
+                for (int i = 0; i < NT; i++) p[i] = __builtin_bit_cast(h2, l32[i]) * __builtin_bit_cast(h2, t32[i]);
+                float sn = num[k][j];
+#pragma unroll
+                for (int i = 0; i < NT; i++)
+                    sn = __builtin_amdgcn_fdot2(p[i], __builtin_bit_cast(h2, cw[j][k + i]), sn, false);
+                num[k][j] = sn;
+            }
+        }
+    };
+
+    // One barrier per window row: each wave's reads of the previous row have
+    // returned (lgkmcnt 0) and its loads of this row have landed (vmcnt 0);
+    // past the barrier the row is complete in BUF and the other buffer is
+    // free for the next row's loads, which then fly during this row's sums.
+    const int ylast = yt + r;
+    auto step = [&](int cur, int yy) {
+        __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+        wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+        if (yy < ylast) issue(cur ^ 1, yy + 1);
+        compute(cur, yy);
+    };
+    issue(0, yt - r);
+    for (int yy = yt - r; yy <= ylast; yy++) step((yy - yt + r) & 1, yy);
+
+    // The denominators through a 64 x QW float matrix (odd pitch), the
+    // quotients into registers, then through LDS so that each disparity row
+    // of 64 pixels is stored contiguously.
+    float *dm = reinterpret_cast<float *>(lbuf);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < T::NSLOT; s++) {
+        const int b = wave + T::NWAVE * s;
+        if (b < T::NBLK) {
+            const int mt = b / (2 * M + 1), nt = mt + b % (2 * M + 1);
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int i = 32 * mt + 8 * (q / 4) + 4 * (dl >> 5) + (q % 4), c = 32 * nt + (dl & 31);
+                dm[i * DMP + c] = macc[s][q];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < AW_NX; k++)
+#pragma unroll
+        for (int j = 0; j < M; j++) {
+            const int px = pxw + 2 * k;
+            num[k][j] = num[k][j] / dm[px * DMP + px - (base + 2 * j) + DT];
+        }
+    __syncthreads();
+    float *tile = reinterpret_cast<float *>(lbuf);
+#pragma unroll
+    for (int k = 0; k < AW_NX; k++)
+#pragma unroll
+        for (int j = 0; j < M; j++) tile[(base + 2 * j) * 65 + pxw + 2 * k] = num[k][j];
+    __syncthreads();
+    for (int i = tid; i < DT * 64; i += T::NTHR) {
+        const int dd = i >> 6, px = i & 63;
+        const int d = d0 + dd, x = x0 + px;
+        if (d < D && x < W) stream_store(&out[(size_t)d * pl + (size_t)yt * W + x], tile[dd * 65 + px]);
+    }
+}
+
+// Disparities per lane for the FP16-volume kernel: the fewest lane-column
+// passes, weighing an output of M = 2 at 0.6 and M = 3 at 0.45 of M = 1
+// (their LDS bytes per output).
+inline int aw_half_m(int D) {
+    int best = 1;
+    double bc = 1e30;
+    for (int m = 1; m <= 3; m++) {
+        const double c = (double)div_up(D, 64 * m) * 64 * m * (m == 1 ? 1.0 : m == 2 ? 0.6 : 0.45);
+        if (c < bc) bc = c, best = m;
+    }
+    return best;
+}
+
+template <int R, int M>
+int aw_half_band_launch(const void *wl, const void *wt, const uint16_t *volh, float *out, int W, int H, int D, int Wp,
+                        int y0, int rows, int r, hipStream_t s) {
+    using T = AwHalf<R, M>;
+    const int nxb = div_up(W, T::TXP), nyb = rows, ndb = div_up(D, T::DT);
+    k_aw_band_h<R, M><<<nxb * nyb * ndb, T::NTHR, 0, s>>>(static_cast<const uint32_t *>(wl),
+                                                         static_cast<const uint32_t *>(wt), volh, out, W, H, D, Wp,
+                                                         y0, r, nxb, nyb, ndb);
+    RTS_LAUNCHED();
+    return RTS_OK;
 }
 
 template <bool HALF, int R>
@@ -817,6 +1112,37 @@ int aw_run(const uint8_t *labL, const uint8_t *labR, const float *vol, float *ou
     return RTS_OK;
 }
 
+template <int R, int M>
+int aw_half_run(const uint8_t *labL, const uint8_t *labR, const uint16_t *volh, float *out, int W, int H, int D,
+                int r, float gc, float gp, const AwPlan &pl, uint8_t *ws, hipStream_t s) {
+    void *wl = ws, *wt = ws + pl.view_bytes;
+    for (int y0 = 0; y0 < H; y0 += pl.band) {
+        const int rows = std::min(pl.band, H - y0);
+        const int nrow = rows * (2 * r + 1);
+        const dim3 grid(pl.Wp / 64, div_up(nrow, 4));
+        const float kc = (float)(-1.4426950408889634 / (double)gc);
+        k_aw_weights_band<true, R>
+            <<<grid, 256, 0, s>>>(labL, static_cast<uint32_t *>(wl), W, H, pl.Wp, y0, nrow, r, kc, gp);
+        RTS_LAUNCHED();
+        k_aw_weights_band<true, R>
+            <<<grid, 256, 0, s>>>(labR, static_cast<uint32_t *>(wt), W, H, pl.Wp, y0, nrow, r, kc, gp);
+        RTS_LAUNCHED();
+        if (int st = aw_half_band_launch<R, M>(wl, wt, volh, out, W, H, D, pl.Wp, y0, rows, r, s)) return st;
+    }
+    return RTS_OK;
+}
+
+template <int M>
+int aw_half_dispatch(const uint8_t *labL, const uint8_t *labR, const uint16_t *volh, float *out, int W, int H, int D,
+                     int r, float gc, float gp, const AwPlan &pl, uint8_t *ws, hipStream_t s) {
+    switch (pl.R) {
+        case 3: return aw_half_run<3, M>(labL, labR, volh, out, W, H, D, r, gc, gp, pl, ws, s);
+        case 5: return aw_half_run<5, M>(labL, labR, volh, out, W, H, D, r, gc, gp, pl, ws, s);
+        case 8: return aw_half_run<8, M>(labL, labR, volh, out, W, H, D, r, gc, gp, pl, ws, s);
+        default: return aw_half_run<16, M>(labL, labR, volh, out, W, H, D, r, gc, gp, pl, ws, s);
+    }
+}
+
 template <bool HALF>
 int aw_dispatch(const uint8_t *labL, const uint8_t *labR, const float *vol, float *out, int W, int H, int D, int r,
                 float gc, float gp, const AwPlan &pl, uint8_t *ws, hipStream_t s) {
@@ -858,6 +1184,32 @@ extern "C" rts_status rts_aw_ws(const uint8_t *lab_ref, const uint8_t *lab_tar, 
     uint8_t *ws = static_cast<uint8_t *>(workspace);
     return half ? aw_dispatch<true>(lab_ref, lab_tar, vol, out, W, H, D, r, gamma_c, gamma_p, pl, ws, s)
                 : aw_dispatch<false>(lab_ref, lab_tar, vol, out, W, H, D, r, gamma_c, gamma_p, pl, ws, s);
+}
+
+extern "C" int rts_aw_half_supported(int W, int H, int D) {
+    if (W < 1 || H < 1 || D < 1 || W % 8 != 0) return 0;
+    // the per-lane cost offsets span a tile's 64*M planes in 32 bits
+    return 2.0 * 64 * 3 * (double)W * (double)H < 4294967296.0 ? 1 : 0;
+}
+
+extern "C" rts_status rts_aw_half_ws(const uint8_t *lab_ref, const uint8_t *lab_tar, const void *vol_half, float *out,
+                                     int W, int H, int D, int r, float gamma_c, float gamma_p, void *workspace,
+                                     size_t workspace_bytes, void *stream) {
+    if (int e = aw_check(static_cast<const float *>(vol_half), out, W, H, D, r, RTS_AW_FP16)) return e;
+    RTS_CHECK(rts_aw_half_supported(W, H, D), "FP16-volume AW needs a width that is a multiple of 8 and "
+              "384*W*H below 2^32 (W %d, H %d)", W, H);
+    RTS_CHECK((reinterpret_cast<uintptr_t>(vol_half) & 15) == 0, "the FP16 volume must be 16-byte aligned");
+    const AwPlan pl = aw_plan(W, H, r, true);
+    RTS_CHECK(workspace && workspace_bytes >= 2 * pl.view_bytes, "AW workspace of %zu bytes is below the %zu needed",
+              workspace_bytes, 2 * pl.view_bytes);
+    hipStream_t s = S(stream);
+    uint8_t *ws = static_cast<uint8_t *>(workspace);
+    const uint16_t *vh = static_cast<const uint16_t *>(vol_half);
+    switch (aw_half_m(D)) {
+        case 1: return aw_half_dispatch<1>(lab_ref, lab_tar, vh, out, W, H, D, r, gamma_c, gamma_p, pl, ws, s);
+        case 2: return aw_half_dispatch<2>(lab_ref, lab_tar, vh, out, W, H, D, r, gamma_c, gamma_p, pl, ws, s);
+        default: return aw_half_dispatch<3>(lab_ref, lab_tar, vh, out, W, H, D, r, gamma_c, gamma_p, pl, ws, s);
+    }
 }
 
 extern "C" rts_status rts_aw(const uint8_t *lab_ref, const uint8_t *lab_tar, const float *vol, float *out, int W,

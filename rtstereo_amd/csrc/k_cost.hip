@@ -16,9 +16,10 @@ constexpr int DCH = 16;   // disparities per block
 enum { AD_GRAY = 0, AD_FLOAT = 1, AD_COLOR = 2 };
 
 // seq/PixelBasedCost.cpp:13-158.  dir 0: tar at x-d (L2R); 1: tar at x+d (R2L).
-template <int KIND>
+// OUT float, or _Float16 (each cost rounded to nearest) for FP16 consumers.
+template <int KIND, typename OUT = float>
 __global__ void __launch_bounds__(XB) k_ad(const void *__restrict__ ref_, const void *__restrict__ tar_,
-                                           float *__restrict__ vol, int W, int H, int D, int dir, int use_trunc,
+                                           OUT *__restrict__ vol, int W, int H, int D, int dir, int use_trunc,
                                            float trunc, int norm) {
     const int x = blockIdx.x * XB + threadIdx.x;
     const int y = blockIdx.y;
@@ -33,7 +34,7 @@ __global__ void __launch_bounds__(XB) k_ad(const void *__restrict__ ref_, const 
         for (int d = d0; d < dend; d++) {
             const int xt = wrap1(dir ? x + d : x - d, W);
             const int t = r - (int)tar[row + xt];
-            vol[d * pl + row + x] = (float)(t < 0 ? -t : t);
+            vol[d * pl + row + x] = (OUT)(float)(t < 0 ? -t : t);
         }
     } else if (KIND == AD_FLOAT) {
         const float *ref = (const float *)ref_, *tar = (const float *)tar_;
@@ -42,7 +43,7 @@ __global__ void __launch_bounds__(XB) k_ad(const void *__restrict__ ref_, const 
             const int xt = wrap1(dir ? x + d : x - d, W);
             float v = r - tar[row + xt];
             if (v < 0) v = -v;
-            vol[d * pl + row + x] = v;
+            vol[d * pl + row + x] = (OUT)v;
         }
     } else {
         const uint8_t *ref = (const uint8_t *)ref_, *tar = (const uint8_t *)tar_;
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(XB) k_ad(const void *__restrict__ ref_, const 
             const int s = (int)sad3(r, pack_rgb(tar + 3 * (row + xt)));
             float v = use_trunc ? (((float)s > trunc) ? trunc : (float)s) : (float)s;
             if (norm) v = third * v;
-            vol[d * pl + row + x] = v;
+            vol[d * pl + row + x] = (OUT)v;
         }
     }
 }
@@ -707,6 +708,18 @@ rts_status rts_ad_float(const float *ref, const float *tar, float *vol, int W, i
 rts_status rts_ad_color(const uint8_t *ref, const uint8_t *tar, float *vol, int W, int H, int D, int dir,
                         int use_trunc, float trunc, int norm, void *stream) {
     return launch_ad(AD_COLOR, ref, tar, vol, W, H, D, dir, use_trunc, trunc, norm, stream);
+}
+
+rts_status rts_ad_color_half(const uint8_t *ref, const uint8_t *tar, void *vol_half, int W, int H, int D, int dir,
+                             int use_trunc, float trunc, int norm, void *stream) {
+    if (int e = check_dims(W, H, D)) return e;
+    RTS_CHECK(dir == RTS_L2R || dir == RTS_R2L, "bad direction %d", dir);
+    RTS_CHECK(H <= 65535, "height %d too large", H);
+    dim3 grid(div_up(W, XB), H, div_up(D, DCH));
+    k_ad<AD_COLOR, _Float16><<<grid, XB, 0, S(stream)>>>(ref, tar, static_cast<_Float16 *>(vol_half), W, H, D, dir,
+                                                          use_trunc, trunc, norm);
+    RTS_LAUNCHED();
+    return RTS_OK;
 }
 
 rts_status rts_census(const uint8_t *img, uint64_t *words, int W, int H, int hr, int wr, void *stream) {

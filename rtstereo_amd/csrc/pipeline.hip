@@ -650,6 +650,9 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
         right_lab = p->lab[1];
     }
     float *cur = p->vol[0], *other = p->vol[1];
+    // FP16 adaptive weights over an FP16 cost volume where the widths allow
+    const bool aw_half =
+        d.aggr == RTS_AGGR_AW && d.aw_precision == RTS_AW_FP16 && rts_aw_half_supported(W, H, D) && d.aw_r <= 16;
     if (d.aggr == RTS_AGGR_CROSS && !views_done)
         for (int v = 0; v < 2; v++) {
             k_pack_rgb<<<(unsigned)div_up((long)W * H, 256), 256, 0, s>>>(rgb[v], p->pcol[v], (size_t)W * H);
@@ -660,7 +663,7 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
                 return e;
             }
         }
-    bool prefixed = false;
+    bool prefixed = false, half_built = false;
     if (p->fused_front) {
         const bool census = census_used, ad = uses(d, RTS_COST_AD);
         if (census && !views_done)
@@ -685,6 +688,14 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
                                              W, H, D, d.ncc_hr, d.ncc_wr, d.lambda1, d.lambda2, d.lambda3, 16,
                                              p->ncc_ws, p->ncc_ws_bytes, s))
             return e;
+    } else if (aw_half && d.cost1 == RTS_COST_AD && !d.cost2) {
+        // the FP16 adaptive weights read an FP16 cost volume: the colour AD
+        // written as half straight into the second volume buffer
+        RTS_MARK(p, "prep", s);
+        if (int e = rts_ad_color_half(rgb[0], rgb[1], other, W, H, D, RTS_L2R, d.ad_trunc_enable, d.ad_trunc,
+                                      d.ad_norm, s))
+            return e;
+        half_built = true;
     } else {
         RTS_MARK(p, "prep", s);
         if (int e = build_cost(p, d.cost1, rgb, gray, cur, s)) return e;
@@ -693,6 +704,8 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
             if (int e = rts_combine(cur, other, d.lambda1, d.lambda2, (size_t)W * H * D, s)) return e;
         }
     }
+    if (aw_half && !half_built)
+        if (int e = rts_volume_to_half(cur, other, (size_t)W * H * D, s)) return e;
     RTS_MARK(p, "cost", s);
     switch (d.aggr) {
         case RTS_AGGR_NONE:
@@ -706,6 +719,12 @@ static rts_status pipeline_launch(rts_pipeline *p, const uint8_t *left_rgb, cons
             std::swap(cur, other);
             break;
         case RTS_AGGR_AW:
+            if (aw_half) {   // FP16 costs in `other`, the result into `cur`
+                if (int e = rts_aw_half_ws(left_lab, right_lab, other, cur, W, H, D, d.aw_r, d.aw_gamma_c,
+                                           d.aw_gamma_p, p->aw_ws, p->aw_ws_bytes, s))
+                    return e;
+                break;
+            }
             if (int e = rts_aw_ws(left_lab, right_lab, cur, other, W, H, D, d.aw_r, d.aw_gamma_c, d.aw_gamma_p,
                                   d.aw_precision, p->aw_ws, p->aw_ws_bytes, s))
                 return e;

@@ -55,11 +55,28 @@ __global__ void k_fill_u8(uint8_t *dst, uint8_t v, size_t n) {
     for (; i < n; i += stride) dst[i] = v;
 }
 
+// FP32 -> FP16 volume, each value rounded to nearest: a thread converts
+// groups of 8 floats into one 16-byte word.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_to_half(const float *__restrict__ src, u32x4 *__restrict__ dst, size_t n8) {
+    const f32x4_t *s4 = reinterpret_cast<const f32x4_t *>(src);
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n8; i += (size_t)gridDim.x * 256) {
+        const f32x4_t a = __builtin_nontemporal_load(s4 + 2 * i), b = __builtin_nontemporal_load(s4 + 2 * i + 1);
+        u32x4 o;
+        o.x = __builtin_bit_cast(unsigned int, h2_t{(_Float16)a.x, (_Float16)a.y});
+        o.y = __builtin_bit_cast(unsigned int, h2_t{(_Float16)a.z, (_Float16)a.w});
+        o.z = __builtin_bit_cast(unsigned int, h2_t{(_Float16)b.x, (_Float16)b.y});
+        o.w = __builtin_bit_cast(unsigned int, h2_t{(_Float16)b.z, (_Float16)b.w});
+        dst[i] = o;
+    }
+}
+
 // Copy probe: a block moves rounds of 256 x CP_U 16-byte words, each thread
 // issuing all CP_U loads of a round before its first store (CP_U loads in
 // flight per lane); loads and stores are non-temporal, as a stream's are.
 constexpr int CP_U = 8;
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_copy16(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, size_t n) {
     constexpr size_t per = 256 * CP_U;
     for (size_t b = (size_t)blockIdx.x * per; b < n; b += (size_t)gridDim.x * per) {
@@ -293,6 +310,17 @@ rts_status rts_vol_to_pm(const float *dm, float *pm, int W, int H, int D, void *
     RTS_CHECK(div_up(P, 64) <= 65535, "volume of %ld pixels is too large to convert", P);
     dim3 grid(div_up(D, 64), div_up(P, 64)), block(64, 4);
     k_transpose<<<grid, block, 0, S(stream)>>>(dm, pm, D, P);
+    RTS_LAUNCHED();
+    return RTS_OK;
+}
+
+rts_status rts_volume_to_half(const float *src, void *dst, size_t n, void *stream) {
+    RTS_CHECK(n % 8 == 0, "volume of %zu floats is not a multiple of 8", n);
+    RTS_CHECK(((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0,
+              "volumes must be 16-byte aligned");
+    const size_t n8 = n / 8;
+    const int blocks = (int)std::min<size_t>((n8 + 255) / 256, 256 * 16);
+    k_to_half<<<blocks, 256, 0, S(stream)>>>(src, static_cast<u32x4 *>(dst), n8);
     RTS_LAUNCHED();
     return RTS_OK;
 }

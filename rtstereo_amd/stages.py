@@ -49,6 +49,22 @@ def ad_color(ref, tar, D, direction=RTS_L2R, trunc=None, norm=False):
     return v.download()
 
 
+def ad_color_half(ref, tar, D, direction=RTS_L2R, trunc=None, norm=False):
+    """Colour AD as float16 [D][H][W] (each float cost rounded to nearest)."""
+    H, W, _ = ref.shape
+    a, b, v = DeviceArray.from_host(ref), DeviceArray.from_host(tar), DeviceArray((D, H, W), np.float16)
+    call("rts_ad_color_half", a.ptr, b.ptr, v.ptr, W, H, D, direction, int(trunc is not None),
+         float(trunc or 0.0), int(norm), None)
+    return v.download()
+
+
+def volume_to_half(vol):
+    a = DeviceArray.from_host(vol.astype(np.float32))
+    h = DeviceArray(vol.shape, np.float16)
+    call("rts_volume_to_half", a.ptr, h.ptr, a.nbytes // 4, None)
+    return h.download()
+
+
 def census(img, hr, wr):
     H, W = img.shape
     nw = ((2 * hr + 1) * (2 * wr + 1) + 63) // 64
@@ -170,6 +186,18 @@ def aw(lab_ref, lab_tar, vol, r, gamma_c, gamma_p, precision=RTS_AW_FP32):
     a, b = DeviceArray.from_host(lab_ref), DeviceArray.from_host(lab_tar)
     v, o = DeviceArray.from_host(vol.astype(np.float32)), _vol(D, H, W)
     call("rts_aw", a.ptr, b.ptr, v.ptr, o.ptr, W, H, D, r, float(gamma_c), float(gamma_p), precision, None)
+    return o.download()
+
+
+def aw_half(lab_ref, lab_tar, vol_half, r, gamma_c, gamma_p):
+    """FP16 adaptive weights from an FP16 cost volume (rts_aw_half_ws)."""
+    D, H, W = vol_half.shape
+    a, b = DeviceArray.from_host(lab_ref), DeviceArray.from_host(lab_tar)
+    v, o = DeviceArray.from_host(vol_half.astype(np.float16)), _vol(D, H, W)
+    nbytes = int(lib.rts_aw_workspace_bytes(W, H, r, 1))
+    ws = DeviceArray((nbytes,), np.uint8)
+    call("rts_aw_half_ws", a.ptr, b.ptr, v.ptr, o.ptr, W, H, D, r, float(gamma_c), float(gamma_p), ws.ptr, nbytes,
+         None)
     return o.download()
 
 

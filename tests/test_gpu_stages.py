@@ -265,6 +265,41 @@ def test_aw_fp16_kernel_paths(W, H, D, r):
     assert np.max(np.abs(got16 - f64)) < 0.05
 
 
+@pytest.mark.parametrize("W,H,D,r", [(128, 24, 64, 5), (136, 30, 70, 3), (128, 40, 100, 8), (256, 40, 192, 16),
+                                     (200, 20, 150, 8), (384, 20, 256, 16), (192, 18, 128, 16)])
+def test_aw_half_volume(W, H, D, r):
+    """FP16 adaptive weights from an FP16 cost volume (one to three
+    disparities per lane: D 64 -> 1, 70/100/128/256 -> 2, 150/192 -> 3;
+    short disparity blocks and column tiles; radii with every staging
+    offset): the FP16 volume is the float costs rounded to nearest (both
+    producers), and the aggregation equals rts_aw's FP16 result bit for bit."""
+    print(f"[case] aw half volume {W}x{H} D={D} r={r}", flush=True)
+    p = gen.make_pair(W, H, D, 8, 3.0, 41)
+    ll, lr = p.lab()
+    c = S.ad_color(p.left, p.right, D, L.RTS_L2R, 40.0)
+    ch = S.ad_color_half(p.left, p.right, D, L.RTS_L2R, 40.0)
+    assert np.array_equal(ch.view(np.uint16), c.astype(np.float16).view(np.uint16))
+    assert np.array_equal(S.volume_to_half(c).view(np.uint16), ch.view(np.uint16))
+    ref16 = S.aw(ll, lr, c, r, 7.0, 36.0, L.RTS_AW_FP16)
+    got = S.aw_half(ll, lr, ch, r, 7.0, 36.0)
+    assert np.isfinite(got).all()
+    assert np.array_equal(got.view(np.uint32), ref16.view(np.uint32))
+    if W * H <= 128 * 24:
+        ys, xs = np.mgrid[0:H, 0:W]
+        f64 = oracle.aw_l2r_f64(ll, lr, oracle.ad_color(p.left, p.right, D, 1, 40.0, False), ys.ravel(), xs.ravel(),
+                                r, 7.0, 36.0).reshape(H, W, D)
+        assert np.max(np.abs(pm(got).astype(np.float64) - f64)) < 0.05
+
+
+def test_aw_half_volume_checks():
+    """Widths that are not a multiple of 8 are refused with an error."""
+    assert L.lib.rts_aw_half_supported(1920, 1080, 192) == 1
+    assert L.lib.rts_aw_half_supported(1922, 1080, 192) == 0
+    ll = np.zeros((8, 20, 3), np.uint8)
+    with pytest.raises(L.RtsError):
+        S.aw_half(ll, ll, np.zeros((4, 8, 20), np.float16), 3, 7.0, 36.0)
+
+
 @pytest.mark.parametrize("Lm,tao", [(40, 30), (60, 255)])
 def test_cross_long_arms(Lm, tao):
     """Arms longer than 17 (beyond the sweep's ring window) take the sweep's
