@@ -92,19 +92,21 @@ __global__ void __launch_bounds__(256) k_aw_weights_band(const uint8_t *__restri
     __syncthreads();
     const float *pr = prox[g];
     const bool ok = yok && x < W;
-    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    uint32_t cpk = 0;   // the centre's Lab bytes
     if (ok) {
         const uint8_t *c = lab + 3 * ((size_t)y * W + x);
-        c0 = c[0];
-        c1 = c[1];
-        c2 = c[2];
+        cpk = (uint32_t)c[0] | (uint32_t)c[1] << 8 | (uint32_t)c[2] << 16;
     }
+    const uint32_t cn = __builtin_amdgcn_udot4(cpk, cpk, 0u, false);   // |c|^2
+    // The squared Lab distance |c - t|^2 = |c|^2 + |t|^2 - 2 c.t from byte
+    // dot products: exact integers below 2^18, so the float square root
+    // sees the value the float sum of the squared differences gives.
     auto weight = [&](int dx) {
         if (!ok || dx > R) return 0.0f;
         const uint32_t t = pix[g][lane + R + 1 + dx];
         if (t == NONE) return 0.0f;
-        const float e0 = c0 - (float)(t & 255u), e1 = c1 - (float)((t >> 8) & 255u), e2 = c2 - (float)(t >> 16);
-        const float dist = __builtin_amdgcn_sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+        const int d2 = (int)__builtin_amdgcn_udot4(t, t, cn, false) - 2 * (int)__builtin_amdgcn_udot4(cpk, t, 0u, false);
+        const float dist = __builtin_amdgcn_sqrtf((float)d2);
         return pr[dx + R] * __builtin_amdgcn_exp2f(dist * kc);
     };
     uint4 *mine = reinterpret_cast<uint4 *>(&outw[g][lane * PG]);
@@ -882,9 +884,14 @@ k_aw_band_h(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb, 
                     macc[s] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, macc[s], 0, 0, 0);
                 }
                 if constexpr (PG % 8 != 0) {
-                    const half4 av = *reinterpret_cast<const half4 *>(ar + PG - 4 + 2 * (dl >> 5));
-                    const half4 bv = *reinterpret_cast<const half4 *>(br + PG - 4 + 2 * (dl >> 5));
-                    macc[s] = __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, macc[s], 0, 0, 0);
+                    // the last 8 taps as a K = 16 step whose upper half is
+                    // zero (lanes 32-63), which takes fewer passes than K = 8
+                    const bool lo = dl < 32;
+                    const u32x4 az = *reinterpret_cast<const u32x4 *>(ar + PG - 4);
+                    const u32x4 bz = *reinterpret_cast<const u32x4 *>(br + PG - 4);
+                    const half8 av = __builtin_bit_cast(half8, lo ? az : u32x4{0u, 0u, 0u, 0u});
+                    const half8 bv = __builtin_bit_cast(half8, lo ? bz : u32x4{0u, 0u, 0u, 0u});
+                    macc[s] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, macc[s], 0, 0, 0);
                 }
             }
         }
