@@ -895,6 +895,10 @@ k_aw_band_h(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb, 
                 }
             }
         }
+        // the next window row's loads go out once this row's cost windows
+        // and matrix operands are read (their LDS writes then meet the
+        // column sums rather than those reads)
+        if (yy < yt + r) issue(which ^ 1, yy + 1);
         // numerators: target row c_k = pxw + 2k - base + DT for k from
         // -(M-1); output (k, j) uses c_(k-j)
         uint32_t tv[AW_NX + M][NT];
@@ -962,14 +966,14 @@ k_aw_band_h(const uint32_t *__restrict__ wlb, const uint32_t *__restrict__ wtb, 
 
     // One barrier per window row: each wave's reads of the previous row have
     // returned (lgkmcnt 0) and its loads of this row have landed (vmcnt 0);
-    // past the barrier the row is complete in BUF and the other buffer is
-    // free for the next row's loads, which then fly during this row's sums.
+    // past the barrier the row is complete in its buffer and the other
+    // buffer is free for the next row's loads, which fly during this row's
+    // sums.
     const int ylast = yt + r;
     auto step = [&](int cur, int yy) {
         __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
         wait_vmcnt<0>();
         __builtin_amdgcn_s_barrier();
-        if (yy < ylast) issue(cur ^ 1, yy + 1);
         compute(cur, yy);
     };
     issue(0, yt - r);
