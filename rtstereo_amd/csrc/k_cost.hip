@@ -59,6 +59,54 @@ __global__ void __launch_bounds__(XB) k_ad(const void *__restrict__ ref_, const 
     }
 }
 
+// The colour AD of k_ad<AD_COLOR> written as half, eight pixels per thread:
+// a block stages one row of both views (packed colour) in LDS and each
+// thread stores its eight costs of a disparity as one 16-byte word (W a
+// multiple of 8).  The same float arithmetic, rounded to half at the end.
+constexpr int ADH_DCH = 32;   // disparities per block
+__global__ void __launch_bounds__(256) k_ad_color_h8(const uint8_t *__restrict__ ref, const uint8_t *__restrict__ tar,
+                                                     _Float16 *__restrict__ vol, int W, int H, int D, int dir,
+                                                     int use_trunc, float trunc, int norm) {
+    // ref and tar rows, one padding word after every 8 (lanes reading at a
+    // stride of 8 pixels then meet no bank conflicts)
+    extern __shared__ uint32_t rows[];
+    const int y = blockIdx.x, d0 = blockIdx.y * ADH_DCH;
+    const int PW = W + W / 8;
+    const size_t row = (size_t)y * W;
+    for (int i = threadIdx.x; i < W; i += 256) {
+        rows[i + (i >> 3)] = pack_rgb(ref + 3 * (row + i));
+        rows[PW + i + (i >> 3)] = pack_rgb(tar + 3 * (row + i));
+    }
+    __syncthreads();
+    const float third = (float)1 / (float)3;
+    const size_t pl = plane(W, H);
+    const int dend = min(D, d0 + ADH_DCH);
+    for (int x = 8 * threadIdx.x; x < W; x += 8 * 256) {
+        uint32_t r[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) r[i] = rows[x + (x >> 3) + i];
+        for (int d = d0; d < dend; d++) {
+            uint32_t o[4];
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {
+                _Float16 h[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int xt = wrap1(dir ? x + i + u + d : x + i + u - d, W);
+                    const int sd = (int)sad3(r[i + u], rows[PW + xt + (xt >> 3)]);
+                    float v = use_trunc ? (((float)sd > trunc) ? trunc : (float)sd) : (float)sd;
+                    if (norm) v = third * v;
+                    h[u] = (_Float16)v;
+                }
+                o[i / 2] = (uint32_t)__builtin_bit_cast(uint16_t, h[0]) | (uint32_t)__builtin_bit_cast(uint16_t, h[1]) << 16;
+            }
+            typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+            const u32x4_t w = {o[0], o[1], o[2], o[3]};
+            __builtin_nontemporal_store(w, reinterpret_cast<u32x4_t *>(vol + (size_t)d * pl + row + x));
+        }
+    }
+}
+
 // seq/CensusTransform.cpp:11-37.  Entry k = (dy+hr)*(2wr+1) + (dx+wr) is set
 // when the centre is strictly brighter than the wrapped neighbour; entries
 // are packed little-endian into 64-bit words, word-major [nw][H][W].
@@ -715,9 +763,15 @@ rts_status rts_ad_color_half(const uint8_t *ref, const uint8_t *tar, void *vol_h
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(dir == RTS_L2R || dir == RTS_R2L, "bad direction %d", dir);
     RTS_CHECK(H <= 65535, "height %d too large", H);
-    dim3 grid(div_up(W, XB), H, div_up(D, DCH));
-    k_ad<AD_COLOR, _Float16><<<grid, XB, 0, S(stream)>>>(ref, tar, static_cast<_Float16 *>(vol_half), W, H, D, dir,
-                                                          use_trunc, trunc, norm);
+    if (W % 8 == 0 && W <= 7168 && (reinterpret_cast<uintptr_t>(vol_half) & 15) == 0) {
+        dim3 grid(H, div_up(D, ADH_DCH));
+        k_ad_color_h8<<<grid, 256, (size_t)(W + W / 8) * 8, S(stream)>>>(ref, tar, static_cast<_Float16 *>(vol_half), W, H, D,
+                                                              dir, use_trunc, trunc, norm);
+    } else {
+        dim3 grid(div_up(W, XB), H, div_up(D, DCH));
+        k_ad<AD_COLOR, _Float16><<<grid, XB, 0, S(stream)>>>(ref, tar, static_cast<_Float16 *>(vol_half), W, H, D,
+                                                              dir, use_trunc, trunc, norm);
+    }
     RTS_LAUNCHED();
     return RTS_OK;
 }
