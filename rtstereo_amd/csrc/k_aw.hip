@@ -41,7 +41,7 @@ constexpr int AW_RMAX = 16;               // largest window radius
 constexpr int AW_TX = 64;                 // weight rows are padded to a multiple of this
 constexpr int AW_NX = 8;                  // pixels per lane
 constexpr int AW_NY = 2;                  // rows per tile
-constexpr size_t AW_WS_CAP = size_t(1) << 30;   // workspace bytes for both views
+constexpr size_t AW_WS_CAP = size_t(2) << 30;   // workspace bytes for both views
 
 // Weight entries per (pixel, window row): tap pairs in FP16, taps in FP32.
 __host__ __device__ inline int aw_ntap(int R, bool half) { return half ? R + 1 : 2 * R + 1; }
@@ -1097,6 +1097,11 @@ AwPlan aw_plan(int W, int H, int r, bool half) {
     band = std::min(band, 4 * 65535 / (2 * r + 1));   // grid.y of the weight kernel
     band -= band % AW_NY;
     if (band < AW_NY) band = AW_NY;
+    // bands of equal height (no short last band with a small grid)
+    if (band < H) {
+        band = div_up(H, div_up(H, band));
+        band += (AW_NY - band % AW_NY) % AW_NY;
+    }
     if (band > H) band = H;
     p.band = band;
     p.view_bytes = row * band;
