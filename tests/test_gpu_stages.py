@@ -267,7 +267,7 @@ def test_aw_fp16_kernel_paths(W, H, D, r):
 
 @pytest.mark.parametrize("W,H,D,r", [(128, 24, 64, 5), (136, 30, 70, 3), (128, 40, 100, 8), (256, 40, 192, 16),
                                      (200, 20, 150, 8), (384, 20, 256, 16), (192, 18, 128, 16), (256, 28, 100, 12),
-                                     (136, 20, 64, 6)])
+                                     (136, 20, 64, 6), (16, 3, 8, 1)])
 def test_aw_half_volume(W, H, D, r):
     """FP16 adaptive weights from an FP16 cost volume (one to three
     disparities per lane: D 64 -> 1, 70/100/128/256 -> 2, 150/192 -> 3;
