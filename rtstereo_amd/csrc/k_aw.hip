@@ -23,6 +23,12 @@
 //   FP16: weights stored as half pairs of neighbouring taps, one packed
 //         product per pair and two v_dot2_f32_f16 (numerator, denominator)
 //         accumulating in float (1.5 VALU per tap).
+//
+// Two FP16 kernels follow the tile kernel: k_aw_band_dma (a float cost
+// volume; one tile row per block, staging by direct loads into LDS, the
+// denominators on the matrix cores, one VALU per tap) and k_aw_band_h (an
+// FP16 cost volume and up to three disparities per lane, so that each weight
+// row read from LDS serves several outputs), which give the same bits.
 #include "rts_internal.h"
 
 #include <hip/hip_fp16.h>
