@@ -59,13 +59,15 @@ __global__ void __launch_bounds__(XB) k_ad(const void *__restrict__ ref_, const 
     }
 }
 
-// The colour AD of k_ad<AD_COLOR> written as half, eight pixels per thread:
-// a block stages one row of both views (packed colour) in LDS and each
-// thread stores its eight costs of a disparity as one 16-byte word (W a
-// multiple of 8).  The same float arithmetic, rounded to half at the end.
+// The colour AD of k_ad<AD_COLOR>, eight pixels per thread: a block stages
+// one row of both views (packed colour) in LDS and each thread stores its
+// eight costs of a disparity as 16-byte words (W a multiple of 8): one word
+// of halves (OUT _Float16, each float cost rounded to nearest) or two of
+// floats.  The same float arithmetic as k_ad.
 constexpr int ADH_DCH = 32;   // disparities per block
+template <typename OUT>
 __global__ void __launch_bounds__(256) k_ad_color_h8(const uint8_t *__restrict__ ref, const uint8_t *__restrict__ tar,
-                                                     _Float16 *__restrict__ vol, int W, int H, int D, int dir,
+                                                     OUT *__restrict__ vol, int W, int H, int D, int dir,
                                                      int use_trunc, float trunc, int norm) {
     // ref and tar rows, one padding word after every 8 (lanes reading at a
     // stride of 8 pixels then meet no bank conflicts)
@@ -86,23 +88,32 @@ __global__ void __launch_bounds__(256) k_ad_color_h8(const uint8_t *__restrict__
 #pragma unroll
         for (int i = 0; i < 8; i++) r[i] = rows[x + (x >> 3) + i];
         for (int d = d0; d < dend; d++) {
-            uint32_t o[4];
+            float c[8];
 #pragma unroll
-            for (int i = 0; i < 8; i += 2) {
-                _Float16 h[2];
-#pragma unroll
-                for (int u = 0; u < 2; u++) {
-                    const int xt = wrap1(dir ? x + i + u + d : x + i + u - d, W);
-                    const int sd = (int)sad3(r[i + u], rows[PW + xt + (xt >> 3)]);
-                    float v = use_trunc ? (((float)sd > trunc) ? trunc : (float)sd) : (float)sd;
-                    if (norm) v = third * v;
-                    h[u] = (_Float16)v;
-                }
-                o[i / 2] = (uint32_t)__builtin_bit_cast(uint16_t, h[0]) | (uint32_t)__builtin_bit_cast(uint16_t, h[1]) << 16;
+            for (int i = 0; i < 8; i++) {
+                const int xt = wrap1(dir ? x + i + d : x + i - d, W);
+                const int sd = (int)sad3(r[i], rows[PW + xt + (xt >> 3)]);
+                float v = use_trunc ? (((float)sd > trunc) ? trunc : (float)sd) : (float)sd;
+                if (norm) v = third * v;
+                c[i] = v;
             }
             typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
-            const u32x4_t w = {o[0], o[1], o[2], o[3]};
-            __builtin_nontemporal_store(w, reinterpret_cast<u32x4_t *>(vol + (size_t)d * pl + row + x));
+            u32x4_t *dst = reinterpret_cast<u32x4_t *>(vol + (size_t)d * pl + row + x);
+            if constexpr (sizeof(OUT) == 2) {
+                uint32_t o[4];
+#pragma unroll
+                for (int i = 0; i < 8; i += 2)
+                    o[i / 2] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)c[i]) |
+                               (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)c[i + 1]) << 16;
+                __builtin_nontemporal_store(u32x4_t{o[0], o[1], o[2], o[3]}, dst);
+            } else {
+                __builtin_nontemporal_store(u32x4_t{__float_as_uint(c[0]), __float_as_uint(c[1]), __float_as_uint(c[2]),
+                                                    __float_as_uint(c[3])},
+                                            dst);
+                __builtin_nontemporal_store(u32x4_t{__float_as_uint(c[4]), __float_as_uint(c[5]), __float_as_uint(c[6]),
+                                                    __float_as_uint(c[7])},
+                                            dst + 1);
+            }
         }
     }
 }
@@ -741,6 +752,10 @@ static rts_status launch_ad(int kind, const void *ref, const void *tar, float *v
         k_ad<AD_GRAY><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, 0, 0.f, 0);
     else if (kind == AD_FLOAT)
         k_ad<AD_FLOAT><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, 0, 0.f, 0);
+    else if (W % 8 == 0 && W <= 7168 && (reinterpret_cast<uintptr_t>(vol) & 15) == 0)
+        k_ad_color_h8<float><<<dim3(H, div_up(D, ADH_DCH)), 256, (size_t)(W + W / 8) * 8, S(stream)>>>(
+            static_cast<const uint8_t *>(ref), static_cast<const uint8_t *>(tar), vol, W, H, D, dir, use_trunc, trunc,
+            norm);
     else
         k_ad<AD_COLOR><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, use_trunc, trunc, norm);
     RTS_LAUNCHED();
@@ -765,8 +780,8 @@ rts_status rts_ad_color_half(const uint8_t *ref, const uint8_t *tar, void *vol_h
     RTS_CHECK(H <= 65535, "height %d too large", H);
     if (W % 8 == 0 && W <= 7168 && (reinterpret_cast<uintptr_t>(vol_half) & 15) == 0) {
         dim3 grid(H, div_up(D, ADH_DCH));
-        k_ad_color_h8<<<grid, 256, (size_t)(W + W / 8) * 8, S(stream)>>>(ref, tar, static_cast<_Float16 *>(vol_half), W, H, D,
-                                                              dir, use_trunc, trunc, norm);
+        k_ad_color_h8<_Float16><<<grid, 256, (size_t)(W + W / 8) * 8, S(stream)>>>(
+            ref, tar, static_cast<_Float16 *>(vol_half), W, H, D, dir, use_trunc, trunc, norm);
     } else {
         dim3 grid(div_up(W, XB), H, div_up(D, DCH));
         k_ad<AD_COLOR, _Float16><<<grid, XB, 0, S(stream)>>>(ref, tar, static_cast<_Float16 *>(vol_half), W, H, D,
