@@ -752,7 +752,7 @@ static rts_status launch_ad(int kind, const void *ref, const void *tar, float *v
         k_ad<AD_GRAY><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, 0, 0.f, 0);
     else if (kind == AD_FLOAT)
         k_ad<AD_FLOAT><<<grid, XB, 0, S(stream)>>>(ref, tar, vol, W, H, D, dir, 0, 0.f, 0);
-    else if (W % 8 == 0 && W <= 7168 && (reinterpret_cast<uintptr_t>(vol) & 15) == 0)
+    else if (W % 8 == 0 && W >= 1024 && W <= 7168 && (reinterpret_cast<uintptr_t>(vol) & 15) == 0)
         k_ad_color_h8<float><<<dim3(H, div_up(D, ADH_DCH)), 256, (size_t)(W + W / 8) * 8, S(stream)>>>(
             static_cast<const uint8_t *>(ref), static_cast<const uint8_t *>(tar), vol, W, H, D, dir, use_trunc, trunc,
             norm);
@@ -778,7 +778,8 @@ rts_status rts_ad_color_half(const uint8_t *ref, const uint8_t *tar, void *vol_h
     if (int e = check_dims(W, H, D)) return e;
     RTS_CHECK(dir == RTS_L2R || dir == RTS_R2L, "bad direction %d", dir);
     RTS_CHECK(H <= 65535, "height %d too large", H);
-    if (W % 8 == 0 && W <= 7168 && (reinterpret_cast<uintptr_t>(vol_half) & 15) == 0) {
+    // (rows of at least 1024 pixels: 128 threads of a block at work)
+    if (W % 8 == 0 && W >= 1024 && W <= 7168 && (reinterpret_cast<uintptr_t>(vol_half) & 15) == 0) {
         dim3 grid(H, div_up(D, ADH_DCH));
         k_ad_color_h8<_Float16><<<grid, 256, (size_t)(W + W / 8) * 8, S(stream)>>>(
             ref, tar, static_cast<_Float16 *>(vol_half), W, H, D, dir, use_trunc, trunc, norm);

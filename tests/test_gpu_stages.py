@@ -293,17 +293,21 @@ def test_aw_half_volume(W, H, D, r):
         assert np.max(np.abs(pm(got).astype(np.float64) - f64)) < 0.05
 
 
-@pytest.mark.parametrize("W,H,D", [(128, 12, 64), (1920, 4, 192), (130, 9, 40)])
+@pytest.mark.parametrize("W,H,D", [(128, 12, 64), (1920, 4, 192), (1032, 5, 100), (130, 9, 40)])
 def test_ad_color_half(W, H, D):
-    """The FP16 colour AD (eight pixels per thread where W is a multiple of 8,
-    the per-pixel kernel otherwise) equals the float AD rounded to half, for
-    both directions, truncation and the 1/3 normalisation."""
+    """The FP16 colour AD (eight pixels per thread where W is a multiple of 8
+    and at least 1024, the per-pixel kernel otherwise) equals the float AD
+    rounded to half, for both directions, truncation and the 1/3
+    normalisation; the float AD of the wide rows equals the oracle's."""
     p = _pair(W, H, D, 43)
     for trunc, norm in ((None, False), (40.0, False), (20.0, True), (None, True)):
         for direction in (L.RTS_L2R, L.RTS_R2L):
             c = S.ad_color(p.left, p.right, D, direction, trunc, norm)
             ch = S.ad_color_half(p.left, p.right, D, direction, trunc, norm)
             assert np.array_equal(ch.view(np.uint16), c.astype(np.float16).view(np.uint16)), (trunc, norm, direction)
+            if W >= 1024:
+                want = oracle.ad_color(p.left, p.right, D, 1 if direction == L.RTS_L2R else -1, trunc, norm)
+                assert np.array_equal(pm(c), want), (trunc, norm, direction)
 
 
 def test_aw_half_volume_checks():
