@@ -140,7 +140,9 @@ __global__ void __launch_bounds__(256) k_aw_weights_band(const uint8_t *__restri
     if (rowok) {
         const uint4 *src = reinterpret_cast<const uint4 *>(outw[g]);
         uint4 *dst = reinterpret_cast<uint4 *>(wb + ((size_t)row * Wp + x0) * PG);
-        for (int i = lane; i < 16 * PG; i += 64) dst[i] = src[i];
+        // (non-temporal: a band's weights exceed the caches and stream out)
+        for (int i = lane; i < 16 * PG; i += 64)
+            __builtin_nontemporal_store(__builtin_bit_cast(u32x4, src[i]), reinterpret_cast<u32x4 *>(dst) + i);
     }
 }
 
